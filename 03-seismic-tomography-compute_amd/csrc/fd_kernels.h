@@ -1,0 +1,47 @@
+// Internal (C++) interface between the C ABI layer and the FD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define STP_STEP_NAN_DETECTED_AND_REPAIRED_DEV 1
+
+namespace stp {
+
+// Per-launch arguments of the leapfrog step (passed by value, lives in SGPRs).
+struct FdStepArgs {
+    const float* ucur;        // current field, all shots
+    float* uprev;             // previous field in, next field out (in place)
+    const float* coef;        // C = v^2 dt^2 (shared by all shots)
+    int64_t ncell;            // nxh*nyh*nzh (per-shot stride)
+    int nxh, nyh, nzh, hw;
+    float cx[5], cy[5], cz[5];  // weights a_k / h^2 per axis (index 1..R used)
+    int sw;                   // sponge width
+    float sigma_over_sw2;     // sponge_damping / sw^2
+    float amp;                // amplitude clamp
+    int damp_prev;            // apply the sponge factor to the previous field
+    int ntz, nty, nxc, nshots;  // tiles along z, y; x chunks; shots
+    // receivers (per-tile CSR lists)
+    int nrcv_pts;             // total list entries (0 = no receivers)
+    int nrcv;                 // receivers per shot
+    const int* blk_rcv_ptr;   // [ntz*nty*nxc + 1]
+    const int64_t* rcv_cell;  // with-halo cell index
+    const int* rcv_id;        // receiver number
+    float* traces;            // [shot][nrcv][trace_stride] or null
+    int64_t trace_stride;
+    int t_col;                // column written this step
+    // source injection of the next step's sample (null = none)
+    const int64_t* src_idx;   // [nshots]
+    const float* w_next;      // &wavelet[t+1] of shot 0; shot s at s*w_stride
+    int64_t w_stride;
+    int* step_status;         // OR-ed with STP_STEP_NAN_DETECTED_AND_REPAIRED
+};
+
+void fd_tiling(int R, int VW, int* tz, int* ty, int* lx);
+hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, hipStream_t st);
+hipError_t fd_launch_inject(float* u, int64_t ncell, const int64_t* src_idx, const float* w, int64_t w_stride,
+                            int nshots, hipStream_t st);
+hipError_t fd_launch_coef(const float* vel, float* coef, int64_t n, float dt2, float damp, hipStream_t st);
+hipError_t fd_launch_vmax(const float* vel, int64_t n, unsigned int* out, hipStream_t st);
+hipError_t fd_launch_sponge(float* d, int nxh, int nyh, int nzh, int hw, int sw, float sg, hipStream_t st);
+
+}  // namespace stp

@@ -1,0 +1,425 @@
+// Acoustic leapfrog FD step for gfx950 (MI355X).
+//
+// One launch advances every shot by one time step, in place on two buffers:
+//   N = U + ((U - D*P) + C*L(U))        written over P
+// which is the reference recurrence
+//   next = 2*cur - prev + v^2 dt^2 * Lap(cur), then prev,cur *= D (sponge)
+// (src/cpp/solver/wave_equation.cpp:262-297, 299-405) with the sponge folded
+// in at read time: the reference damps `previous` after each step, here P is
+// stored undamped and D is applied when it is read.  L is evaluated from
+// neighbour-minus-centre differences so that its FP32 rounding error scales
+// with |L| rather than with |U|.
+//
+// Work decomposition (x is the slowest axis, z contiguous):
+//   * a workgroup owns a TZ x TY tile of the (y,z) plane and marches along x
+//     over a chunk of LX planes;
+//   * each thread owns VW consecutive z cells (one float4 when VW = 4) in TYR
+//     rows and keeps a register queue of the 2R+1 planes x-R..x+R of its own
+//     cells (x-derivative taps come from registers);
+//   * the centre plane x is staged in LDS with an R-cell halo in y and z
+//     (y- and z-derivative taps come from LDS); corners are never loaded
+//     (star stencil); LDS is double buffered, one barrier per plane;
+//   * the loads for plane x+1 (halo, queue front x+1+R, P, C) are issued
+//     before the barrier of plane x so they overlap its arithmetic;
+//   * workgroup ids are remapped so each XCD gets a contiguous range of tiles
+//     (neighbouring tiles share that XCD's L2 for their halos).
+// Sponge: D = min(Dx(plane), Dyz(cell)) because the reference factor is a
+// monotone function of min(distance) — Dyz is computed once per thread.
+// Finite guards run only when the result is non-finite (a non-finite input
+// always makes it so), which reproduces the reference's eager checks.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fd_kernels.h"
+
+namespace stp {
+
+template <int VW> struct Vec;
+template <> struct Vec<1> {
+    using T = float;
+    __device__ static inline float get(const T& v, int) { return v; }
+    __device__ static inline void set(T& v, int, float x) { v = x; }
+    __device__ static inline T zero() { return 0.f; }
+};
+template <> struct Vec<4> {
+    using T = float4;
+    __device__ static inline float get(const T& v, int j) {
+        return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+    }
+    __device__ static inline void set(T& v, int j, float x) {
+        if (j == 0) v.x = x; else if (j == 1) v.y = x; else if (j == 2) v.z = x; else v.w = x;
+    }
+    __device__ static inline T zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+};
+
+__device__ __forceinline__ bool finitef(float x) { return __builtin_isfinite(x); }
+
+// Sponge factor from a distance-to-interior-face d (reference formula).
+__device__ __forceinline__ float sponge_of(int d, int sw, float sigma_over_sw2) {
+    if (d < sw && d >= 0) {
+        const float e = (float)(sw - d);
+        return 1.0f - sigma_over_sw2 * e * e;
+    }
+    return 1.0f;
+}
+
+// distance component along one axis, as in apply_sponge_layer: min over the
+// low face (i - hw) and high face (nh - hw - i - 1) when within the band.
+__device__ __forceinline__ int sponge_dist(int i, int nh, int hw, int sw) {
+    int d = sw;
+    if (i < hw + sw) d = min(d, i - hw);
+    if (i >= nh - hw - sw) d = min(d, nh - hw - i - 1);
+    return d;
+}
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    // hardware dispatches workgroup b to XCD (b % 8); give each XCD a
+    // contiguous range of logical tiles.
+    const int xcd = b & 7, k = b >> 3, q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
+}
+
+template <int R, int VW, int TZ, int TY, int TYR, int LX>
+__global__ void __launch_bounds__((TZ / VW) * (TY / TYR))
+fd_step_kernel(const FdStepArgs a) {
+    using V = Vec<VW>;
+    using vec = typename V::T;
+    constexpr int NTZ = TZ / VW;              // threads along z
+    constexpr int NTY = TY / TYR;             // threads along y
+    constexpr int NT = NTZ * NTY;
+    constexpr int HV = (R + VW - 1) / VW;     // halo vectors per z side
+    constexpr int LW = NTZ + 2 * HV;          // LDS row width (vectors)
+    constexpr int LH = TY + 2 * R;            // LDS rows
+    constexpr int NHALO = 2 * R * NTZ + 2 * HV * TY;
+    constexpr int NH = (NHALO + NT - 1) / NT; // halo items per thread
+
+    __shared__ vec lds[2][LH][LW];
+
+    const int tid = threadIdx.x;
+    const int tz = tid % NTZ, ty = tid / NTZ;
+
+    // ---- decode workgroup -> (shot, x chunk, y tile, z tile)
+    const int lb = xcd_remap(blockIdx.x, gridDim.x);
+    int rem = lb;
+    const int zt = rem % a.ntz; rem /= a.ntz;
+    const int yt = rem % a.nty; rem /= a.nty;
+    const int xc = rem % a.nxc; rem /= a.nxc;
+    const int shot = rem;
+
+    const int hw = a.hw, nyh = a.nyh, nzh = a.nzh, nxh = a.nxh;
+    const int64_t plane = (int64_t)nyh * nzh;
+    const int zb = hw + zt * TZ, yb = hw + yt * TY;
+    const int x0 = hw + xc * LX;
+    const int x1 = min(x0 + LX, nxh - hw);
+
+    const float* __restrict__ U = a.ucur + (int64_t)shot * a.ncell;
+    float* __restrict__ P = a.uprev + (int64_t)shot * a.ncell;
+    const float* __restrict__ C = a.coef;
+
+    // own cells
+    const int z = zb + tz * VW;
+    const bool zvalid = z < nzh - hw;
+    bool rvalid[TYR];
+    int64_t roff[TYR];   // offset of own vector within a plane
+#pragma unroll
+    for (int r = 0; r < TYR; ++r) {
+        const int y = yb + ty * TYR + r;
+        rvalid[r] = zvalid && (y < nyh - hw);
+        roff[r] = rvalid[r] ? (int64_t)y * nzh + z : 0;
+    }
+
+    // halo items: LDS slot and in-plane global offset (or -1 = zero fill)
+    int hslot[NH];
+    int64_t hoff[NH];
+#pragma unroll
+    for (int m = 0; m < NH; ++m) {
+        const int i = tid + m * NT;
+        int ly = 0, lz = 0;
+        bool use = i < NHALO;
+        if (use) {
+            if (i < 2 * R * NTZ) {           // y halo rows (interior columns)
+                const int row = i / NTZ, col = i % NTZ;
+                ly = row < R ? row : row + TY;
+                lz = HV + col;
+            } else {                          // z halo columns (interior rows)
+                const int j = i - 2 * R * NTZ;
+                const int row = j / (2 * HV), col = j % (2 * HV);
+                ly = R + row;
+                lz = col < HV ? col : col + NTZ;
+            }
+        }
+        hslot[m] = use ? ly * LW + lz : -1;
+        const int gy = yb - R + ly;
+        const int gz = zb + (lz - HV) * VW;
+        const bool inb = use && gy >= 0 && gy < nyh && gz >= 0 && gz + VW <= nzh;
+        hoff[m] = inb ? (int64_t)gy * nzh + gz : -1;
+    }
+
+    // sponge: Dyz per own cell (constant along x)
+    const float sg = a.sigma_over_sw2;
+    const int sw = a.sw;
+    float dyz[TYR][VW];
+#pragma unroll
+    for (int r = 0; r < TYR; ++r) {
+        const int y = yb + ty * TYR + r;
+        const int dy = sponge_dist(y, nyh, hw, sw);
+#pragma unroll
+        for (int j = 0; j < VW; ++j) {
+            const int dz = sponge_dist(z + j, nzh, hw, sw);
+            dyz[r][j] = sponge_of(min(dy, dz), sw, sg);
+        }
+    }
+
+    auto ldv = [&](const float* base, int64_t off) -> vec {
+        return *reinterpret_cast<const vec*>(base + off);
+    };
+
+    // ---- warm-up: queue planes x0-R .. x0+R-1 into q[1..2R]
+    vec q[2 * R + 1][TYR];
+#pragma unroll
+    for (int k = 1; k <= 2 * R; ++k) {
+        const int64_t pb = (int64_t)(x0 - R + k - 1) * plane;
+#pragma unroll
+        for (int r = 0; r < TYR; ++r) q[k][r] = rvalid[r] ? ldv(U, pb + roff[r]) : V::zero();
+    }
+    // prefetch for plane x0
+    vec qn[TYR], pn[TYR], cn[TYR], hn[NH];
+    {
+        const int64_t pb = (int64_t)x0 * plane;
+        const int64_t pf = (int64_t)(x0 + R) * plane;
+#pragma unroll
+        for (int r = 0; r < TYR; ++r) {
+            qn[r] = rvalid[r] ? ldv(U, pf + roff[r]) : V::zero();
+            pn[r] = rvalid[r] ? ldv(P, pb + roff[r]) : V::zero();
+            cn[r] = rvalid[r] ? ldv(C, pb + roff[r]) : V::zero();
+        }
+#pragma unroll
+        for (int m = 0; m < NH; ++m) hn[m] = hoff[m] >= 0 ? ldv(U, pb + hoff[m]) : V::zero();
+    }
+
+    const float amp = a.amp;
+    const bool damp_prev = a.damp_prev != 0;
+    bool guard_hit = false;
+
+    int it = 0;
+    for (int x = x0; x < x1; ++x, ++it) {
+        // rotate queue: q[k] <- q[k+1], q[2R] <- plane x+R
+#pragma unroll
+        for (int k = 0; k < 2 * R; ++k)
+#pragma unroll
+            for (int r = 0; r < TYR; ++r) q[k][r] = q[k + 1][r];
+        vec pc[TYR], cc[TYR];
+#pragma unroll
+        for (int r = 0; r < TYR; ++r) { q[2 * R][r] = qn[r]; pc[r] = pn[r]; cc[r] = cn[r]; }
+
+        // stage plane x in LDS
+        vec(*L)[LW] = lds[it & 1];
+#pragma unroll
+        for (int r = 0; r < TYR; ++r) L[R + ty * TYR + r][HV + tz] = q[R][r];
+#pragma unroll
+        for (int m = 0; m < NH; ++m)
+            if (hslot[m] >= 0) (&L[0][0])[hslot[m]] = hn[m];
+
+        // prefetch plane x+1
+        if (x + 1 < x1) {
+            const int64_t pb = (int64_t)(x + 1) * plane;
+            const int64_t pf = (int64_t)(x + 1 + R) * plane;
+#pragma unroll
+            for (int r = 0; r < TYR; ++r) {
+                qn[r] = rvalid[r] ? ldv(U, pf + roff[r]) : V::zero();
+                pn[r] = rvalid[r] ? ldv(P, pb + roff[r]) : V::zero();
+                cn[r] = rvalid[r] ? ldv(C, pb + roff[r]) : V::zero();
+            }
+#pragma unroll
+            for (int m = 0; m < NH; ++m) hn[m] = hoff[m] >= 0 ? ldv(U, pb + hoff[m]) : V::zero();
+        }
+        __syncthreads();
+
+        const float dxs = sponge_of(sponge_dist(x, nxh, hw, sw), sw, sg);
+        const int64_t pb = (int64_t)x * plane;
+#pragma unroll
+        for (int r = 0; r < TYR; ++r) {
+            const int lr = R + ty * TYR + r;
+            // z window: HV vectors left, own, HV right
+            vec zw[2 * HV + 1];
+#pragma unroll
+            for (int h = 0; h < HV; ++h) {
+                zw[h] = L[lr][tz + h];
+                zw[HV + 1 + h] = L[lr][HV + tz + 1 + h];
+            }
+            zw[HV] = q[R][r];
+            // y column
+            vec yc[2 * R + 1];
+#pragma unroll
+            for (int k = -R; k <= R; ++k) {
+                const int rr = r + k;
+                if (k == 0) yc[R] = q[R][r];
+                else if (rr >= 0 && rr < TYR) yc[R + k] = q[R][rr];
+                else yc[R + k] = L[lr + k][HV + tz];
+            }
+            vec out;
+#pragma unroll
+            for (int j = 0; j < VW; ++j) {
+                const float u = V::get(q[R][r], j);
+                float lap = 0.f;
+#pragma unroll
+                for (int k = 1; k <= R; ++k) {
+                    const float xs = (V::get(q[R + k][r], j) - u) + (V::get(q[R - k][r], j) - u);
+                    const float ys = (V::get(yc[R + k], j) - u) + (V::get(yc[R - k], j) - u);
+                    const int zp = HV * VW + j + k, zm = HV * VW + j - k;
+                    const float zs = (V::get(zw[zp / VW], zp % VW) - u) + (V::get(zw[zm / VW], zm % VW) - u);
+                    lap = fmaf(a.cx[k], xs, lap);
+                    lap = fmaf(a.cy[k], ys, lap);
+                    lap = fmaf(a.cz[k], zs, lap);
+                }
+                const float p = V::get(pc[r], j);
+                const float d = damp_prev ? fminf(dxs, dyz[r][j]) : 1.0f;
+                const float c = V::get(cc[r], j);
+                float nv = u + (fmaf(-d, p, u) + c * lap);
+                if (!finitef(nv) && rvalid[r]) {
+                    // reference guard order (wave_equation.cpp:345-397): a
+                    // non-finite centre or previous value gives 0, a non-finite
+                    // neighbour keeps the centre value (both unclamped), a
+                    // non-finite result keeps the centre value (clamped).
+                    guard_hit = true;
+                    const int64_t ci = pb + (roff[r] - z) + (z + j);
+                    bool ok_n = true;
+#pragma unroll
+                    for (int k = 1; k <= R; ++k) {
+                        ok_n &= finitef(U[ci + k * plane]) && finitef(U[ci - k * plane]);
+                        ok_n &= finitef(U[ci + k * nzh]) && finitef(U[ci - k * nzh]);
+                        ok_n &= finitef(U[ci + k]) && finitef(U[ci - k]);
+                    }
+                    if (!(finitef(u) && finitef(p))) nv = 0.f;
+                    else if (!ok_n) nv = u;
+                    else nv = fminf(fmaxf(u, -amp), amp);
+                } else {
+                    nv = fminf(fmaxf(nv, -amp), amp);
+                }
+                V::set(out, j, nv);
+            }
+            if (rvalid[r]) *reinterpret_cast<vec*>(P + pb + roff[r]) = out;
+        }
+    }
+
+    if (guard_hit && a.step_status) atomicOr(a.step_status, STP_STEP_NAN_DETECTED_AND_REPAIRED_DEV);
+
+    // ---- epilogue: receivers sample the new field, then the next source
+    // sample is injected (the reference injects at the start of the next step).
+    if (a.nrcv_pts == 0 && a.src_idx == nullptr) return;
+    __syncthreads();
+    const int tile = lb % (a.ntz * a.nty * a.nxc);
+    if (a.traces) {
+        const int pbeg = a.blk_rcv_ptr[tile], pend = a.blk_rcv_ptr[tile + 1];
+        for (int p = pbeg + tid; p < pend; p += NT) {
+            const int64_t ci = a.rcv_cell[p];
+            const int rid = a.rcv_id[p];
+            a.traces[((int64_t)shot * a.nrcv + rid) * a.trace_stride + a.t_col] = P[ci];
+        }
+    }
+    if (a.src_idx && a.w_next) {
+        __syncthreads();   // receivers read the pre-injection value
+        if (tid == 0) {
+            const int64_t s = a.src_idx[shot];
+            const int sx = (int)(s / plane), sy = (int)((s / nzh) % nyh), sz = (int)(s % nzh);
+            if (sx >= x0 && sx < x1 && sy >= yb && sy < yb + TY && sz >= zb && sz < zb + TZ) {
+                const float w = a.w_next[(int64_t)shot * a.w_stride];
+                if (finitef(w)) P[s] += w;
+            }
+        }
+    }
+}
+
+// Injection of the first source sample of a propagate call (1 thread per shot).
+__global__ void fd_inject_kernel(float* u, int64_t ncell, const int64_t* src_idx, const float* w,
+                                 int64_t w_stride, int nshots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nshots) return;
+    const float v = w[(int64_t)s * w_stride];
+    if (__builtin_isfinite(v)) u[(int64_t)s * ncell + src_idx[s]] += v;
+}
+
+// C = v^2 dt^2 * damp, with the reference's velocity substitution.
+__global__ void fd_coef_kernel(const float* vel, float* coef, int64_t n, float dt2, float damp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = vel[i];
+    if (!__builtin_isfinite(v) || v <= 0.f) v = 1500.f;
+    coef[i] = v * v * dt2 * damp;
+}
+
+// max finite velocity (values are >= 0 after the filter, so the float bits
+// order like unsigned ints and atomicMax on the bit pattern is exact).
+__global__ void fd_vmax_kernel(const float* vel, int64_t n, unsigned int* out) {
+    unsigned int best = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = vel[i];
+        if (__builtin_isfinite(v) && v > 0.f) best = max(best, __float_as_uint(v));
+    }
+    for (int o = 32; o > 0; o >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+}
+
+__global__ void fd_sponge_kernel(float* d, int nxh, int nyh, int nzh, int hw, int sw, float sg) {
+    const int64_t n = (int64_t)nxh * nyh * nzh;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ix = (int)(i / ((int64_t)nyh * nzh));
+    const int iy = (int)((i / nzh) % nyh);
+    const int iz = (int)(i % nzh);
+    const int dd = min(sponge_dist(ix, nxh, hw, sw), min(sponge_dist(iy, nyh, hw, sw), sponge_dist(iz, nzh, hw, sw)));
+    d[i] = sponge_of(dd, sw, sg);
+}
+
+// ------------------------------------------------------------------ launchers
+template <int R, int VW, int TZ, int TY, int TYR, int LX>
+static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, hipStream_t st) {
+    constexpr int NT = (TZ / VW) * (TY / TYR);
+    hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX>), dim3(nblocks), dim3(NT), 0, st, a0);
+    return hipGetLastError();
+}
+
+void fd_tiling(int R, int VW, int* tz, int* ty, int* lx) {
+    (void)R;
+    if (VW == 4) { *tz = 64; *ty = 16; *lx = 64; }
+    else { *tz = 64; *ty = 8; *lx = 64; }
+}
+
+hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, hipStream_t st) {
+    const int nblocks = a.ntz * a.nty * a.nxc * a.nshots;
+    if (R == 4 && VW == 4) return launch_cfg<4, 4, 64, 16, 2, 64>(a, nblocks, st);
+    if (R == 4 && VW == 1) return launch_cfg<4, 1, 64, 8, 2, 64>(a, nblocks, st);
+    if (R == 1 && VW == 4) return launch_cfg<1, 4, 64, 16, 2, 64>(a, nblocks, st);
+    if (R == 1 && VW == 1) return launch_cfg<1, 1, 64, 8, 2, 64>(a, nblocks, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t fd_launch_inject(float* u, int64_t ncell, const int64_t* src_idx, const float* w, int64_t w_stride,
+                            int nshots, hipStream_t st) {
+    hipLaunchKernelGGL(fd_inject_kernel, dim3((nshots + 63) / 64), dim3(64), 0, st, u, ncell, src_idx, w, w_stride,
+                       nshots);
+    return hipGetLastError();
+}
+
+hipError_t fd_launch_coef(const float* vel, float* coef, int64_t n, float dt2, float damp, hipStream_t st) {
+    hipLaunchKernelGGL(fd_coef_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vel, coef, n, dt2, damp);
+    return hipGetLastError();
+}
+
+hipError_t fd_launch_vmax(const float* vel, int64_t n, unsigned int* out, hipStream_t st) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(fd_vmax_kernel, dim3((unsigned)blocks), dim3(256), 0, st, vel, n, out);
+    return hipGetLastError();
+}
+
+hipError_t fd_launch_sponge(float* d, int nxh, int nyh, int nzh, int hw, int sw, float sg, hipStream_t st) {
+    const int64_t n = (int64_t)nxh * nyh * nzh;
+    hipLaunchKernelGGL(fd_sponge_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d, nxh, nyh, nzh, hw,
+                       sw, sg);
+    return hipGetLastError();
+}
+
+}  // namespace stp

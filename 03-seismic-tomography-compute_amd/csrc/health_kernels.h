@@ -1,0 +1,10 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stp {
+hipError_t health_count(const double* d, int64_t n, int64_t* out_host, hipStream_t st);
+hipError_t health_clamp(double* d, int64_t n, double lo, double hi, hipStream_t st);
+hipError_t health_repair(double* d, int64_t count, int64_t nx, int64_t ny, int64_t nz, int hw, double repl,
+                         hipStream_t st);
+}  // namespace stp

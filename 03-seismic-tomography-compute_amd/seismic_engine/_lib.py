@@ -1,0 +1,100 @@
+"""ctypes binding of libseismic_hip.so (the C ABI declared in include/seismic_hip.h).
+
+The library must exist: there is no fallback path.  Every call checks the
+returned status and raises SeismicError with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("STP_LIB", PKG_DIR / "lib" / "libseismic_hip.so"))
+
+
+class SeismicError(RuntimeError):
+    pass
+
+
+class FdConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("halo_width", C.c_int32), ("order", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("sponge_width", C.c_int32), ("sponge_damping", C.c_double),
+        ("amplitude_limit", C.c_double), ("cfl_safety_factor", C.c_double),
+        ("nshots", C.c_int32),
+    ]
+
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_f64 = C.c_double
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+SIGNATURES: dict[str, list] = {
+    "stp_version": [],
+    "stp_device_alloc": [C.POINTER(_vp), _i64],
+    "stp_device_free": [_vp],
+    "stp_memcpy_h2d": [_vp, _vp, _i64, _vp],
+    "stp_memcpy_d2h": [_vp, _vp, _i64, _vp],
+    "stp_memcpy_d2d": [_vp, _vp, _i64, _vp],
+    "stp_memset_d": [_vp, C.c_int, _i64, _vp],
+    "stp_stream_sync": [_vp],
+    "stp_device_count": [C.POINTER(C.c_int)],
+    "stp_fd_plan_create": [C.POINTER(FdConfig), C.POINTER(_vp)],
+    "stp_fd_plan_destroy": [_vp],
+    "stp_fd_set_receivers": [_vp, _vp, _i32],
+    "stp_fd_set_sources": [_vp, _vp, _i32],
+    "stp_fd_coefficients": [_vp, _vp, _vp, C.POINTER(_f64), _vp],
+    "stp_fd_verify_cfl": [_vp, _f64, C.POINTER(_i32)],
+    "stp_fd_propagate": [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp],
+    "stp_fd_sponge_factors": [_vp, _vp, _vp],
+    "stp_count_nonfinite_f64": [_vp, _i64, C.POINTER(_i64), _vp],
+    "stp_clamp_field_f64": [_vp, _i64, _f64, _f64, _vp],
+    "stp_repair_field_nan_f64": [_vp, _i64, _i64, _i64, _i64, _i32, _f64, _vp],
+}
+_RESTYPES = {"stp_last_error": C.c_char_p}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the library once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SeismicError(
+                f"{LIB_PATH} is missing: build it with `python __graft_entry__.py build` "
+                "(there is no CPU fallback)")
+        h = C.CDLL(str(LIB_PATH))
+        h.stp_last_error.restype = C.c_char_p
+        h.stp_last_error.argtypes = []
+        for name, args in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = h
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().stp_last_error().decode(errors="replace")
+        raise SeismicError(f"{what or 'seismic_hip'} failed ({status}): {msg}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t) -> int:
+    """Device/host address of a torch tensor (0 for None)."""
+    return 0 if t is None else int(t.data_ptr())
+
+
+def stream_handle(device=None) -> int:
+    import torch
+    return int(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,181 @@
+"""Benchmark: 8th-order acoustic FD propagation on one MI355X.
+
+Workload (one bench step): a full forward propagation on a 512^3 grid
+(FP32, dx = 10 m, dt = 1 ms, 2000 time steps, 20-cell sponge), velocity =
+seeded random field smoothed by a Gaussian of sigma = 20 cells and scaled to
+1500-4500 m/s, Ricker 15 Hz source at the centre, 256 receivers on the
+surface line iy = 256 (ix = 0, 2, ..., 510) sampled every step.  Inputs are
+resident in HBM before timing starts.
+
+Prints ONE JSON line: Gcell-updates/s, the FD step kernel against the HBM
+roofline (16 algorithmic bytes per cell-update), and the FP64 CPU oracle
+timed on this host as a reported baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "03-seismic-tomography-compute_amd"))
+
+METRIC = "FD Gcell-updates/s + %HBM roofline; time-to-1e-6 CG: learned-PCG vs Jacobi"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = 16.0   # read u^t, u^{t-1}, c; write u^{t+1}
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def pmc_traffic(n: int) -> tuple[float | None, str | None]:
+    """HBM bytes per FD launch from the committed rocprofv3 PMC summary."""
+    p = ROOT / "profiles" / f"fd_step_kernel_{n}cube_pmc.md"
+    if not p.exists():
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
+
+
+def cpu_baseline(threads: int, steps: int = 100):
+    """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
+    on the 128^3 two-layer case for `steps` of its 500 steps."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import numpy as np
+    from oracle import fd as ofd
+    from seismic_engine import synthetic
+    n = (128, 128, 128)
+    c = ofd.cfg(n, hw=4, order=8, sponge_width=20, sponge_damping=0.015)
+    v = synthetic.pad_halo(synthetic.two_layer_velocity(*n, split_iz=64), 4)
+    w = synthetic.ricker(15.0, 1e-3, steps)
+    rcv = synthetic.surface_receivers(*n, 4)
+    t0 = time.perf_counter()
+    cur, _, tr, res = ofd.forward(c, v, w, steps, rcv=rcv)
+    dt = time.perf_counter() - t0
+    assert res.completed_successfully == 1
+    return 128**3 * steps / dt / 1e9, dt, cur, tr, v, w, rcv
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                    or min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from seismic_engine import FDPropagator, synthetic
+
+    dev = torch.device("cuda:0")
+    n, nt, hw = args.n, args.nt, 4
+    log(f"building {n}^3 velocity (seed 2026, sigma 20) on the GPU")
+    vel = synthetic.smooth_random_velocity(n, n, n, seed=2026, sigma=20.0, device=dev, as_tensor=True)
+    fd = FDPropagator(n, n, n, order=8, sponge_width=20, sponge_damping=0.015, device=dev)
+    fd.set_velocity(vel)
+    del vel
+    if not fd.verify_cfl():
+        raise SystemExit("CFL check failed")
+    fd.set_sources([fd.index(n // 2, n // 2, n // 2)])
+    rcv = synthetic.line_receivers(n, n, n, hw, iy=n // 2, step=max(1, n // 256))
+    fd.set_receivers(rcv)
+    w = torch.as_tensor(synthetic.ricker(15.0, 1e-3, nt), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    def one():
+        fd.reset()
+        return fd.propagate(nt, w)
+
+    for i in range(args.warmup):
+        one()
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup} done")
+
+    stream = torch.cuda.current_stream(dev)
+    ev_prop = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        fd.reset()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        tr = fd.propagate(nt, w)
+        e1.record(stream)
+        ev_prop.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prop_ms = [a.elapsed_time(b) for a, b in ev_prop]
+    log(f"timed {args.steps} propagations: wall {wall:.3f} s, propagate {prop_ms} ms")
+    assert torch.isfinite(tr).all()
+
+    cells = float(n) ** 3
+    value = cells * nt * args.steps / wall / 1e9
+    launch_s = sum(prop_ms) / 1e3 / (args.steps * nt)
+    achieved = BYTES_PER_CELL * cells / launch_s / 1e9
+    traffic, traffic_src = pmc_traffic(n)
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"fd8_forward_{n}cube_{nt}steps_{len(rcv)}rcv", "grid": [n, n, n],
+                   "order": 8, "time_steps": nt, "receivers": int(len(rcv)), "sponge": [20, 0.015],
+                   "velocity": "smooth random seed 2026 sigma 20, 1500-4500 m/s", "source": "Ricker 15 Hz centre",
+                   "step": "one full forward propagation"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "fd_step_kernel<4,4,64,16,2,64>", "bytes_per_cell_update": BYTES_PER_CELL,
+                     "avg_launch_us": round(launch_s * 1e6, 2), "traffic_source": traffic_src},
+    }
+
+    if not args.no_cpu_baseline:
+        del fd
+        torch.cuda.empty_cache()
+        steps = 100
+        log(f"cpu baseline: FP64 oracle 128^3, {steps} steps, {args.cpu_threads} threads")
+        gcs, sec, cur_o, tr_o, v_h, w_np, rcv1 = cpu_baseline(args.cpu_threads, steps)
+        result["cpu_baseline"] = {
+            "value": round(gcs, 5), "unit": "Gcell-updates/s", "cores": args.cpu_threads, "kind": "port",
+            "sample": f"FP64 oracle (reference 4-pass step restated, OpenMP over x), 128^3 two-layer, "
+                      f"order 8, first {steps} of 500 steps, {sec:.1f} s"}
+        # FP32 GPU vs FP64 oracle on that same sample
+        g = FDPropagator(128, 128, 128, order=8, sponge_width=20, sponge_damping=0.015, device=dev)
+        g.set_velocity(v_h)
+        g.set_sources([g.index(64, 64, 64)])
+        g.set_receivers(rcv1)
+        trg = g.propagate(steps, w_np)
+        torch.cuda.synchronize()
+        e = float(np.linalg.norm(g.cur[0].cpu().numpy() - cur_o) / np.linalg.norm(cur_o))
+        et = float(np.linalg.norm(trg[0].cpu().numpy() - tr_o) / np.linalg.norm(tr_o))
+        result["accuracy"] = {"case": f"128^3 two-layer, {steps} steps", "field_rel_l2_vs_fp64": e,
+                              "traces_rel_l2_vs_fp64": et}
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,133 @@
+/*
+ * seismic_hip.h — C ABI of the MI355X (gfx950) seismic compute library.
+ *
+ * Every entry point returns an int status: 0 = success, negative = error
+ * (stp_last_error() returns the message, thread-local).  No C++ exception
+ * crosses this boundary.  Pointers named *_dev are device pointers owned by
+ * the caller; pointers named *_host are host pointers owned by the caller.
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ * Handles are not thread-safe: use one handle per host thread / stream.
+ *
+ * Wavefield layout (all FD entry points): flat with-halo arrays,
+ *   idx = ix*nyh*nzh + iy*nzh + iz,   n?h = n? + 2*halo_width,
+ * z contiguous, exactly the reference's CartesianGrid::local_index_with_halo
+ * (src/cpp/core/grid.cpp:45-49).  Batched shots are stacked: shot s starts at
+ * s*nxh*nyh*nzh.
+ */
+#ifndef SEISMIC_HIP_H
+#define SEISMIC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Step status codes, same values as the reference's StepResult
+ * (src/cpp/solver/wave_equation.h:24-29). */
+enum {
+    STP_STEP_OK = 0,
+    STP_STEP_NAN_DETECTED_AND_REPAIRED = 1,
+    STP_STEP_CFL_VIOLATION = 2,
+    STP_STEP_FATAL_DIVERGENCE = 3
+};
+
+/* Error codes */
+enum {
+    STP_OK = 0,
+    STP_ERR_INVALID = -1,   /* bad argument / shape */
+    STP_ERR_HIP = -2,       /* HIP runtime error */
+    STP_ERR_ALLOC = -3,     /* device or host allocation failed */
+    STP_ERR_STATE = -4      /* call out of order (e.g. no receivers set) */
+};
+
+const char* stp_last_error(void);
+int stp_version(void);
+
+/* ---------------------------------------------------------------- device memory
+ * Thin helpers so host C++ above this ABI never calls HIP itself. */
+int stp_device_alloc(void** ptr_dev, int64_t bytes);
+int stp_device_free(void* ptr_dev);
+int stp_memcpy_h2d(void* dst_dev, const void* src_host, int64_t bytes, void* stream);
+int stp_memcpy_d2h(void* dst_host, const void* src_dev, int64_t bytes, void* stream);
+int stp_memcpy_d2d(void* dst_dev, const void* src_dev, int64_t bytes, void* stream);
+int stp_memset_d(void* dst_dev, int value, int64_t bytes, void* stream);
+int stp_stream_sync(void* stream);
+int stp_device_count(int* n);
+
+/* ---------------------------------------------------------------- FD propagator
+ * Replaces WaveEquationSolver's stencil/sponge/rotation/injection loop
+ * (src/cpp/solver/wave_equation.cpp:39-152, 262-405) for order 2 (reference
+ * 7-point) and order 8 (radius-4 star stencil). */
+typedef struct {
+    int64_t nx, ny, nz;         /* interior cells, GridParams::n?_global (grid.h:9-20) */
+    int32_t halo_width;         /* GridParams::halo_width; >= order/2 */
+    int32_t order;              /* 2 or 8 */
+    double dx, dy, dz;          /* GridParams::d? */
+    double dt;                  /* SolverConfig::dt (config.h:9-19) */
+    int32_t sponge_width;       /* SolverConfig::sponge_width */
+    double sponge_damping;      /* SolverConfig::sponge_damping */
+    double amplitude_limit;     /* WaveEquationSolver amplitude_limit_ (1e10) */
+    double cfl_safety_factor;   /* FaultTolerance cfl_safety_factor_ (0.49) */
+    int32_t nshots;             /* wavefields advanced per launch (>= 1) */
+} stp_fd_config;
+
+typedef struct stp_fd_plan stp_fd_plan;
+
+int stp_fd_plan_create(const stp_fd_config* cfg, stp_fd_plan** out);
+int stp_fd_plan_destroy(stp_fd_plan* plan);
+
+/* Receivers: with-halo flat indices of interior cells (same set for every shot).
+ * New: the reference samples no receivers. */
+int stp_fd_set_receivers(stp_fd_plan* plan, const int64_t* idx_host, int32_t nrcv);
+/* Sources: one with-halo flat index per shot (reference: the interior centre,
+ * wave_equation.cpp:120-124). */
+int stp_fd_set_sources(stp_fd_plan* plan, const int64_t* idx_host, int32_t nshots);
+
+/* C = v^2 dt^2 per with-halo cell, with the reference's substitutions
+ * (non-finite or <= 0 velocity -> 1500, wave_equation.cpp:375-378) and its CFL
+ * sub-cycling damp factor folded in (wave_equation.cpp:316-331, 383-387).
+ * vmax_host receives max finite velocity (compute_local_max_velocity,
+ * wave_equation.cpp:435-443).  Synchronises `stream`. */
+int stp_fd_coefficients(stp_fd_plan* plan, const float* vel_dev, float* coef_dev,
+                        double* vmax_host, void* stream);
+
+/* verify_cfl (wave_equation.cpp:445-451, fault_tolerance.cpp:49-70): per-axis
+ * vmax*dt/h <= cfl_safety_factor; for order 8 also the leapfrog stability
+ * bound of the wider stencil.  *ok = 1 if propagation may start. */
+int stp_fd_verify_cfl(const stp_fd_plan* plan, double vmax, int32_t* ok);
+
+/* Advance nsteps leapfrog steps in place.
+ *   u_cur_dev  : current field u^t (all shots), pre-injection
+ *   u_prev_dev : previous field; stored undamped unless prev_damped = 1
+ *                (then it already carries the sponge factor, as the
+ *                reference's `previous` buffer does)
+ * Step s (t = t_offset + s) injects wavelet_dev[shot*nw + t] at the shot's
+ * source when t < nw (non-finite samples skipped), then advances, then samples
+ * receivers into traces_dev[(shot*nrcv + r)*trace_stride + s].  wavelet_dev
+ * and traces_dev may be NULL.  step_status_dev[s] (may be NULL) is OR-ed with
+ * STP_STEP_NAN_DETECTED_AND_REPAIRED when a finite guard fired in step s.
+ * After an odd nsteps the two buffers have swapped roles: the new current
+ * field is in u_prev_dev.  Asynchronous on `stream`. */
+int stp_fd_propagate(stp_fd_plan* plan, float* u_cur_dev, float* u_prev_dev, const float* coef_dev,
+                     int32_t nsteps, int32_t prev_damped, const float* wavelet_dev, int32_t nw,
+                     int32_t t_offset, float* traces_dev, int32_t trace_stride,
+                     int32_t* step_status_dev, void* stream);
+
+/* Sponge factor D per with-halo cell (1 outside the sponge, and in the halo),
+ * apply_sponge_layer (wave_equation.cpp:262-297). */
+int stp_fd_sponge_factors(const stp_fd_plan* plan, float* d_dev, void* stream);
+
+/* ---------------------------------------------------------------- field health
+ * GPU versions of FaultTolerance::count_nan_inf / clamp_field /
+ * repair_field_nan (fault_tolerance.cpp:72-157), as bound by
+ * check_field_health / clamp_field_safe / repair_field_nan (bindings.cpp:275-301). */
+int stp_count_nonfinite_f64(const double* data_dev, int64_t n, int64_t* count_host, void* stream);
+int stp_clamp_field_f64(double* data_dev, int64_t n, double lo, double hi, void* stream);
+int stp_repair_field_nan_f64(double* data_dev, int64_t count, int64_t nx, int64_t ny, int64_t nz,
+                             int32_t halo_width, double replace_value, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
