@@ -161,3 +161,24 @@ def cell_index(n: tuple[int, int, int], hw: int, ix: int, iy: int, iz: int) -> i
     """With-halo flat index of interior cell (ix, iy, iz)."""
     nx, ny, nz = n
     return ((ix + hw) * (ny + 2 * hw) + (iy + hw)) * (nz + 2 * hw) + (iz + hw)
+
+
+def ray_endpoints(nrays: int, n: tuple[int, int, int], h: float = 1.0, seed: int = 2026) -> np.ndarray:
+    """(nrays, 2, 3) float64 source/receiver points on opposite faces of the
+    box [0, n*h]^3.  Ray r joins face pair r % 3 (x, y, z faces); source on
+    the low face, receiver on the high face, the other two coordinates of
+    each point uniform from the counter-based generator."""
+    ext = np.asarray(n, dtype=np.float64) * h
+    u = np.stack([uniform(seed, 0, nrays, stream=10 + k) for k in range(4)], axis=1)
+    pts = np.zeros((nrays, 2, 3))
+    axis = np.arange(nrays) % 3
+    for a in range(3):
+        sel = axis == a
+        o = [b for b in range(3) if b != a]
+        pts[sel, 0, a] = 0.0
+        pts[sel, 1, a] = ext[a]
+        pts[sel, 0, o[0]] = u[sel, 0] * ext[o[0]]
+        pts[sel, 0, o[1]] = u[sel, 1] * ext[o[1]]
+        pts[sel, 1, o[0]] = u[sel, 2] * ext[o[0]]
+        pts[sel, 1, o[1]] = u[sel, 3] * ext[o[1]]
+    return pts

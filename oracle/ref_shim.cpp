@@ -1,0 +1,115 @@
+// Test-only driver: extern "C" entry points over the reference's own
+// translation units (compiled from /root/reference by oracle/build.py into
+// oracle/_ref/libref.so).  Only functions whose reference code is safe to run
+// are exposed:
+//   * CSRMatrix::build_from_triplets / spmv / add_diagonal / scale /
+//     frobenius_norm (src/cpp/sparse/csr_matrix.cpp:25-193), only for inputs
+//     where every row keeps >= 8 entries after merging, because
+//     pad_to_alignment (csr_matrix.cpp:223-264) writes past its buffers
+//     otherwise; spmv_transpose is not exposed (it accumulates into
+//     uninitialised memory, aligned_alloc.h:61-77);
+//   * RTMEngine::apply_image_condition / finalize_image / reset_image
+//     (src/cpp/rtm/rtm_engine.cpp:24-57, 170-190);
+//   * load_config (src/cpp/core/config.cpp:9-96);
+//   * CartesianGrid index math (src/cpp/core/grid.cpp:13-49).
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "core/config.h"
+#include "core/grid.h"
+#include "rtm/rtm_engine.h"
+#include "sparse/csr_matrix.h"
+
+using namespace seismic;
+
+extern "C" {
+
+// returns nullptr when some row would keep fewer than 8 entries
+void* ref_csr_build(int64_t rows, int64_t cols, int64_t n, const int64_t* r, const int64_t* c, const double* v) {
+    std::vector<std::set<int64_t>> uniq(rows > 0 ? rows : 0);
+    for (int64_t i = 0; i < n; ++i)
+        if (r[i] >= 0 && r[i] < rows && c[i] >= 0 && c[i] < cols) uniq[r[i]].insert(c[i]);
+    for (const auto& s : uniq)
+        if (s.size() < 8) return nullptr;
+    std::vector<SparseEntry> e(n);
+    for (int64_t i = 0; i < n; ++i) e[i] = {r[i], c[i], v[i]};
+    auto* m = new CSRMatrix();
+    m->build_from_triplets(rows, cols, e);
+    return m;
+}
+
+int64_t ref_csr_nnz(void* h) { return static_cast<CSRMatrix*>(h)->nnz(); }
+
+void ref_csr_arrays(void* h, int64_t* rp, int64_t* ci, double* val) {
+    auto* m = static_cast<CSRMatrix*>(h);
+    std::memcpy(rp, m->row_ptr().data(), sizeof(int64_t) * (m->rows() + 1));
+    std::memcpy(ci, m->col_ind().data(), sizeof(int64_t) * m->nnz());
+    std::memcpy(val, m->values().data(), sizeof(double) * m->nnz());
+}
+
+void ref_csr_spmv(void* h, const double* x, int64_t nx, double* y) {
+    auto* m = static_cast<CSRMatrix*>(h);
+    AlignedVector<double> xv(nx);
+    for (int64_t i = 0; i < nx; ++i) xv[i] = x[i];
+    AlignedVector<double> yv = m->spmv(xv);
+    std::memcpy(y, yv.data(), sizeof(double) * yv.size());
+}
+
+void ref_csr_add_diagonal(void* h, double a) { static_cast<CSRMatrix*>(h)->add_diagonal(a); }
+void ref_csr_scale(void* h, double a) { static_cast<CSRMatrix*>(h)->scale(a); }
+double ref_csr_frobenius(void* h) { return static_cast<CSRMatrix*>(h)->frobenius_norm(); }
+void ref_csr_free(void* h) { delete static_cast<CSRMatrix*>(h); }
+
+void* ref_rtm_create(int64_t nx, int64_t ny, int64_t nz, int hw) {
+    return new RTMEngine(nx, ny, nz, 10.0, 10.0, 10.0, hw);
+}
+void ref_rtm_apply(void* h, const double* f, const double* a, int64_t nyh, int64_t nzh, int64_t step) {
+    static_cast<RTMEngine*>(h)->apply_image_condition(f, a, nyh, nzh, step);
+}
+void ref_rtm_finalize(void* h) { static_cast<RTMEngine*>(h)->finalize_image(); }
+void ref_rtm_reset(void* h) { static_cast<RTMEngine*>(h)->reset_image(); }
+void ref_rtm_result(void* h, double* img, double* ill, double* refl) {
+    auto* e = static_cast<RTMEngine*>(h);
+    const auto& r = e->result();
+    std::memcpy(img, r.image.data(), sizeof(double) * r.image.size());
+    std::memcpy(ill, r.illumination.data(), sizeof(double) * r.illumination.size());
+    std::memcpy(refl, r.reflectivity.data(), sizeof(double) * r.reflectivity.size());
+}
+void ref_rtm_free(void* h) { delete static_cast<RTMEngine*>(h); }
+
+// load_config: 21 numeric fields in a fixed order, plus the two strings.
+// Returns 0, or 1 when the reference threw (bad number).
+int ref_load_config(const char* path, double* out, char* misfit_type, char* output_dir, int cap) {
+    try {
+        TomographyConfig c = load_config(path);
+        const double v[] = {c.solver.frequency, c.solver.dt, (double)c.solver.nt, c.solver.water_depth,
+                            c.solver.seafloor_gradient, (double)c.solver.sponge_width, c.solver.sponge_damping,
+                            (double)c.solver.snapshot_interval, c.model.vp_min, c.model.vp_max, c.model.vs_min,
+                            c.model.vs_max, c.model.rho_min, c.model.rho_max, c.model.initial_vp,
+                            c.model.initial_vs, c.model.initial_rho, (double)c.inversion.max_iterations,
+                            c.inversion.step_length, c.inversion.gradient_smoothing_sigma,
+                            c.inversion.regularization_lambda, c.inversion.misfit_tolerance,
+                            (double)c.inversion.gradient_precondition, (double)c.inversion.line_search_max,
+                            c.inversion.line_search_factor};
+        std::memcpy(out, v, sizeof v);
+        std::strncpy(misfit_type, c.inversion.misfit_type.c_str(), cap - 1);
+        misfit_type[cap - 1] = 0;
+        std::strncpy(output_dir, c.solver.output_dir.c_str(), cap - 1);
+        output_dir[cap - 1] = 0;
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+int64_t ref_grid_index_with_halo(int64_t nx, int64_t ny, int64_t nz, int hw, int64_t ix, int64_t iy, int64_t iz) {
+    GridParams p{nx, ny, nz, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, hw};
+    const int coords[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
+    CartesianGrid g(p, coords, dims);
+    return g.local_index_with_halo(ix, iy, iz);
+}
+
+}  // extern "C"
