@@ -13,29 +13,20 @@
 
 #include "fd_kernels.h"
 #include "health_kernels.h"
+#include "capi_common.h"
 #include "seismic_hip.h"
 
+namespace stp {
+std::string& last_error() {
+    thread_local std::string e;
+    return e;
+}
+}  // namespace stp
+
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-int hip_fail(hipError_t e, const char* where) {
-    g_err = std::string(where) + ": " + hipGetErrorString(e);
-    return STP_ERR_HIP;
-}
-
-#define STP_HIP(call)                                   \
-    do {                                                \
-        hipError_t _e = (call);                         \
-        if (_e != hipSuccess) return hip_fail(_e, #call); \
-    } while (0)
-
-inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+using stp::fail;
+using stp::hip_fail;
+using stp::S;
 
 int fd_weights(int order, double* a) {
     if (order == 2) { a[0] = -2.0; a[1] = 1.0; return 1; }
@@ -67,7 +58,7 @@ struct stp_fd_plan {
 
 extern "C" {
 
-const char* stp_last_error(void) { return g_err.c_str(); }
+const char* stp_last_error(void) { return stp::last_error().c_str(); }
 int stp_version(void) { return 100; }
 
 int stp_device_alloc(void** p, int64_t bytes) {

@@ -1,0 +1,368 @@
+// C ABI of the tomography solve (declarations: include/seismic_hip.h).
+//
+// stp_csr holds A (FP32 values, int32 columns/row offsets) and an explicit
+// A^T built on the device, plus a lazily sized solver workspace.  Solvers
+// keep their scalars on the device and only synchronise with the host every
+// CHECK iterations to read the convergence flag.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "capi_common.h"
+#include "learned_precond.h"
+#include "seismic_hip.h"
+#include "sparse_kernels.h"
+
+using stp::fail;
+using stp::hip_fail;
+using stp::S;
+
+namespace {
+constexpr int CHECK = 8;   // iterations between host reads of the done flag
+
+template <typename T>
+hipError_t grow(T** p, size_t* cap, size_t n) {
+    if (*cap >= n && *p) return hipSuccess;
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+}  // namespace
+
+struct stp_csr {
+    int rows = 0, cols = 0;
+    int64_t nnz = 0;
+    int *rp = nullptr, *ci = nullptr, *trp = nullptr, *tci = nullptr;
+    float *val = nullptr, *tval = nullptr;
+    int W = 64, WT = 64;   // lanes per row for A and A^T
+    // workspace
+    double *r = nullptr, *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
+    double *part4 = nullptr, *part5 = nullptr;
+    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr;
+    double *dp = nullptr, *dap = nullptr, *dz = nullptr, *dr = nullptr, *ddinv = nullptr;
+    stp::SolverScalars* sc = nullptr;
+    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0;
+    size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;
+};
+
+static void csr_free(stp_csr* h) {
+    void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
+                  h->p, h->q, h->s, h->z, h->sp, h->dinv, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
+    for (void* p : ps)
+        if (p) hipFree(p);
+}
+
+static int finish_csr(stp_csr* h, hipStream_t st) {
+    STP_HIP(hipMalloc(&h->trp, sizeof(int) * ((size_t)h->cols + 1)));
+    STP_HIP(hipMalloc(&h->tci, sizeof(int) * (h->nnz ? h->nnz : 1)));
+    STP_HIP(hipMalloc(&h->tval, sizeof(float) * (h->nnz ? h->nnz : 1)));
+    STP_HIP(stp::csr_transpose(h->rows, h->cols, h->nnz, h->rp, h->ci, h->val, h->trp, h->tci, h->tval, st));
+    h->W = stp::spmv_lanes(h->rows ? (double)h->nnz / h->rows : 0.0);
+    h->WT = stp::spmv_lanes(h->cols ? (double)h->nnz / h->cols : 0.0);
+    return STP_OK;
+}
+
+extern "C" {
+
+int stp_csr_create(int64_t rows, int64_t cols, int64_t nnz, const int32_t* rp, const int32_t* ci, const float* val,
+                   stp_csr** out, void* stream) {
+    if (!out || rows < 0 || cols < 0 || nnz < 0 || rows > 0x7ffffffe || cols > 0x7ffffffe || nnz > 0x7fffffff ||
+        !rp || (nnz > 0 && (!ci || !val)))
+        return fail(STP_ERR_INVALID, "stp_csr_create: bad argument");
+    *out = nullptr;
+    stp_csr* h = new (std::nothrow) stp_csr();
+    if (!h) return fail(STP_ERR_ALLOC, "stp_csr_create: out of host memory");
+    h->rows = (int)rows; h->cols = (int)cols; h->nnz = nnz;
+    hipStream_t st = S(stream);
+    hipError_t e;
+    if ((e = hipMalloc(&h->rp, sizeof(int) * (rows + 1))) != hipSuccess ||
+        (e = hipMalloc(&h->ci, sizeof(int) * (nnz ? nnz : 1))) != hipSuccess ||
+        (e = hipMalloc(&h->val, sizeof(float) * (nnz ? nnz : 1))) != hipSuccess ||
+        (e = hipMemcpyAsync(h->rp, rp, sizeof(int) * (rows + 1), hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (nnz && (e = hipMemcpyAsync(h->ci, ci, sizeof(int) * nnz, hipMemcpyDeviceToDevice, st)) != hipSuccess) ||
+        (nnz && (e = hipMemcpyAsync(h->val, val, sizeof(float) * nnz, hipMemcpyDeviceToDevice, st)) != hipSuccess)) {
+        csr_free(h);
+        delete h;
+        return hip_fail(e, "stp_csr_create");
+    }
+    const int rc = finish_csr(h, st);
+    if (rc != STP_OK) { csr_free(h); delete h; return rc; }
+    *out = h;
+    return STP_OK;
+}
+
+int stp_csr_from_rays(int64_t nrays, const int64_t* n, double h_cell, const double* pts, stp_csr** out,
+                      void* stream) {
+    if (!out || !n || nrays < 0 || nrays > 0x7ffffffe || (nrays > 0 && !pts) || !(h_cell > 0) || n[0] < 1 ||
+        n[1] < 1 || n[2] < 1 || n[0] * n[1] * n[2] > 0x7ffffffe)
+        return fail(STP_ERR_INVALID, "stp_csr_from_rays: bad argument");
+    *out = nullptr;
+    hipStream_t st = S(stream);
+    stp_csr* h = new (std::nothrow) stp_csr();
+    if (!h) return fail(STP_ERR_ALLOC, "stp_csr_from_rays: out of host memory");
+    h->rows = (int)nrays;
+    h->cols = (int)(n[0] * n[1] * n[2]);
+    int* counts = nullptr;
+    hipError_t e = hipSuccess;
+    int last_count = 0, last_off = 0;
+    do {
+        if ((e = hipMalloc(&counts, sizeof(int) * (nrays + 1))) != hipSuccess) break;
+        if ((e = hipMemsetAsync(counts, 0, sizeof(int) * (nrays + 1), st)) != hipSuccess) break;
+        if ((e = stp::rays_count(nrays, pts, n, h_cell, counts, st)) != hipSuccess) break;
+        if ((e = hipMalloc(&h->rp, sizeof(int) * (nrays + 1))) != hipSuccess) break;
+        if ((e = stp::exclusive_scan_i32(counts, h->rp, (int)nrays + 1, st)) != hipSuccess) break;
+        if ((e = hipMemcpy(&last_off, h->rp + nrays, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) break;
+        (void)last_count;
+        h->nnz = last_off;
+        if ((e = hipMalloc(&h->ci, sizeof(int) * (h->nnz ? h->nnz : 1))) != hipSuccess) break;
+        if ((e = hipMalloc(&h->val, sizeof(float) * (h->nnz ? h->nnz : 1))) != hipSuccess) break;
+        if ((e = stp::rays_fill(nrays, pts, n, h_cell, h->rp, h->ci, h->val, st)) != hipSuccess) break;
+        e = hipStreamSynchronize(st);
+    } while (0);
+    if (counts) hipFree(counts);
+    if (e != hipSuccess) { csr_free(h); delete h; return hip_fail(e, "stp_csr_from_rays"); }
+    const int rc = finish_csr(h, st);
+    if (rc != STP_OK) { csr_free(h); delete h; return rc; }
+    *out = h;
+    return STP_OK;
+}
+
+int stp_csr_destroy(stp_csr* h) {
+    if (!h) return STP_OK;
+    csr_free(h);
+    delete h;
+    return STP_OK;
+}
+
+int stp_csr_shape(const stp_csr* h, int64_t* rows, int64_t* cols, int64_t* nnz) {
+    if (!h || !rows || !cols || !nnz) return fail(STP_ERR_INVALID, "stp_csr_shape: null argument");
+    *rows = h->rows; *cols = h->cols; *nnz = h->nnz;
+    return STP_OK;
+}
+
+int stp_csr_export(const stp_csr* h, int32_t transpose, int32_t* rp, int32_t* ci, float* val, void* stream) {
+    if (!h || !rp || (h->nnz && (!ci || !val))) return fail(STP_ERR_INVALID, "stp_csr_export: null argument");
+    hipStream_t st = S(stream);
+    const int n = transpose ? h->cols : h->rows;
+    STP_HIP(hipMemcpyAsync(rp, transpose ? h->trp : h->rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToDevice, st));
+    if (h->nnz) {
+        STP_HIP(hipMemcpyAsync(ci, transpose ? h->tci : h->ci, sizeof(int) * h->nnz, hipMemcpyDeviceToDevice, st));
+        STP_HIP(hipMemcpyAsync(val, transpose ? h->tval : h->val, sizeof(float) * h->nnz, hipMemcpyDeviceToDevice, st));
+    }
+    return STP_OK;
+}
+
+int stp_csr_spmv(const stp_csr* h, int32_t transpose, const float* x, float* y, void* stream) {
+    if (!h || !x || !y) return fail(STP_ERR_INVALID, "stp_csr_spmv: null argument");
+    if (transpose)
+        STP_HIP(stp::spmv_f32(h->WT, h->cols, h->trp, h->tci, h->tval, x, y, nullptr, nullptr, S(stream)));
+    else
+        STP_HIP(stp::spmv_f32(h->W, h->rows, h->rp, h->ci, h->val, x, y, nullptr, nullptr, S(stream)));
+    return STP_OK;
+}
+
+int stp_csr_spmv_f64(const stp_csr* h, int32_t transpose, const double* x, double* y, void* stream) {
+    if (!h || !x || !y) return fail(STP_ERR_INVALID, "stp_csr_spmv_f64: null argument");
+    if (transpose)
+        STP_HIP(stp::spmv_f64(h->WT, h->cols, h->trp, h->tci, h->tval, x, y, 1.0, nullptr, 0.0, nullptr, nullptr,
+                              S(stream)));
+    else
+        STP_HIP(stp::spmv_f64(h->W, h->rows, h->rp, h->ci, h->val, x, y, 1.0, nullptr, 0.0, nullptr, nullptr,
+                              S(stream)));
+    return STP_OK;
+}
+
+// ------------------------------------------------------------------ square CG / Jacobi-PCG
+int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int32_t jacobi, double* x,
+           double* history, stp_solve_info* info, void* stream) {
+    stp_csr* h = const_cast<stp_csr*>(hc);
+    if (!h || !b || !x || max_iter < 0) return fail(STP_ERR_INVALID, "stp_cg: bad argument");
+    if (h->rows != h->cols) return fail(STP_ERR_INVALID, "stp_cg: matrix is not square");
+    hipStream_t st = S(stream);
+    const int n = h->rows;
+    const int nbd = stp::cg_blocks(n);
+    STP_HIP(grow(&h->dp, &h->c_dp, n));
+    STP_HIP(grow(&h->dap, &h->c_dap, n));
+    STP_HIP(grow(&h->dz, &h->c_dz, n));
+    STP_HIP(grow(&h->dr, &h->c_dr, n));
+    STP_HIP(grow(&h->ddinv, &h->c_ddinv, n));
+    STP_HIP(grow(&h->part0, &h->c_p0, nbd));
+    STP_HIP(grow(&h->part1, &h->c_p1, nbd));
+    STP_HIP(grow(&h->sc, &h->c_sc, 1));
+    double* dinv = nullptr;
+    if (jacobi) {
+        STP_HIP(stp::launch_diag_inv(n, h->rp, h->ci, h->val, h->ddinv, st));
+        dinv = h->ddinv;
+    }
+    double* z = jacobi ? h->dz : h->dr;
+    STP_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
+    STP_HIP(hipMemcpyAsync(h->dr, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    if (jacobi) {
+        // z = D^-1 r via the update kernel with alpha = 0 is not available before init; use a copy + scale
+        STP_HIP(hipMemsetAsync(h->sc, 0, sizeof(stp::SolverScalars), st));
+        STP_HIP(hipMemsetAsync(h->dap, 0, sizeof(double) * n, st));
+        STP_HIP(stp::launch_cg_update(h->sc, n, x, h->dr, h->dap, h->dap, dinv, h->dz, h->part0, h->part1, st));
+    } else {
+        STP_HIP(stp::launch_dot_partial(nullptr, n, h->dr, h->dr, h->part0, st));
+        STP_HIP(stp::launch_dot_partial(nullptr, n, h->dr, h->dr, h->part1, st));
+    }
+    STP_HIP(hipMemcpyAsync(h->dp, z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    STP_HIP(stp::launch_cg_init(h->sc, h->part0, h->part1, nbd, st));
+    int* done = &h->sc->done;
+    int it = 0;
+    stp::SolverScalars hs;
+    std::memset(&hs, 0, sizeof hs);
+    for (; it < max_iter; ++it) {
+        STP_HIP(stp::spmv_f64(h->W, n, h->rp, h->ci, h->val, h->dp, h->dap, 1.0, nullptr, 0.0, nullptr, done, st));
+        STP_HIP(stp::launch_dot_partial(done, n, h->dp, h->dap, h->part0, st));
+        STP_HIP(stp::launch_cg_alpha(h->sc, h->part0, nbd, st));
+        STP_HIP(stp::launch_cg_update(h->sc, n, x, h->dr, h->dp, h->dap, dinv, h->dz, h->part0, h->part1, st));
+        STP_HIP(stp::launch_cg_beta(h->sc, h->part0, h->part1, nbd, tol, it, history, st));
+        STP_HIP(stp::launch_cg_direction(h->sc, n, z, h->dp, st));
+        if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
+            STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+            STP_HIP(hipStreamSynchronize(st));
+            if (hs.done) break;
+        }
+    }
+    STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+    STP_HIP(hipStreamSynchronize(st));
+    if (info) {
+        info->iterations = hs.iters;
+        info->converged = hs.done == 1;
+        info->rel_residual = hs.rel;
+        info->true_rel_residual = hs.rel;
+        info->replacements = 0;
+    }
+    return STP_OK;
+}
+
+// ------------------------------------------------------------------ CGLS
+// CGLS on (A^T A + lam I) x = A^T b, x0 = 0.  precond 0: none; 1: Jacobi
+// (1 / column sums of squares + lam); 2: learned network, flexible variant
+// (beta = z.(s - s_prev) / gamma_prev).  Vectors p, q, s, z are FP32 (the SpMV
+// A p accumulates in FP32); x and r are FP64 and every dot product is FP64.
+// When the recursive residual reaches tol the true residual is recomputed in
+// FP64 (r = b - A x, s = A^T r - lam x); if it is still above tol the
+// iteration restarts from it (iterative refinement of the FP32 inner solve).
+int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t max_iter, int32_t precond,
+             void* precond_handle, double* x, double* history, stp_solve_info* info, void* stream) {
+    stp_csr* h = const_cast<stp_csr*>(hc);
+    if (!h || !b || !x || max_iter < 0 || lam < 0 || precond < 0 || precond > 2)
+        return fail(STP_ERR_INVALID, "stp_cgls: bad argument");
+    if (precond == 2 && !precond_handle) return fail(STP_ERR_INVALID, "stp_cgls: learned preconditioner handle missing");
+    hipStream_t st = S(stream);
+    const int rows = h->rows, cols = h->cols;
+    const int nbq = std::max(stp::spmv_blocks(rows, h->W), 1);
+    const int nbs = std::max(stp::spmv_blocks(cols, h->WT), 1);
+    const int nbc = std::max((cols + 255) / 256, 1);
+    STP_HIP(grow(&h->r, &h->c_r, rows));
+    STP_HIP(grow(&h->p, &h->c_p, cols));
+    STP_HIP(grow(&h->q, &h->c_q, rows));
+    STP_HIP(grow(&h->s, &h->c_s, cols));
+    STP_HIP(grow(&h->z, &h->c_z, cols));
+    STP_HIP(grow(&h->sp, &h->c_sp, cols));
+    STP_HIP(grow(&h->dinv, &h->c_dinv, cols));
+    STP_HIP(grow(&h->part0, &h->c_p0, nbq));
+    STP_HIP(grow(&h->part1, &h->c_p1, nbc));
+    STP_HIP(grow(&h->part2, &h->c_p2, nbs));
+    STP_HIP(grow(&h->part3, &h->c_p3, nbs));
+    STP_HIP(grow(&h->part4, &h->c_p4, nbc));
+    STP_HIP(grow(&h->part5, &h->c_p5, nbc));
+    STP_HIP(grow(&h->sc, &h->c_sc, 1));
+    const bool flex = precond == 2;
+    float* dinv = nullptr;
+    if (precond == 1) {
+        STP_HIP(stp::launch_normal_diag(cols, h->trp, h->tval, lam, h->dinv, st));
+        dinv = h->dinv;
+    }
+    int* done = &h->sc->done;
+
+    // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s
+    auto start = [&](int set_s0) -> int {
+        STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z, h->part2,
+                                     h->part3, nullptr, st));
+        if (flex) {
+            int rc = stp::lpc_apply(precond_handle, h->s, h->z, nullptr, st);
+            if (rc != STP_OK) return rc;
+            STP_HIP(hipMemsetAsync(h->sp, 0, sizeof(float) * cols, st));
+            STP_HIP(stp::launch_flex_dots(nullptr, cols, h->z, h->s, h->sp, h->part4, h->part5, st));
+            STP_HIP(stp::launch_cgls_init(h->sc, h->part4, nbc, h->part3, nbs, set_s0, st));
+        } else {
+            STP_HIP(stp::launch_cgls_init(h->sc, h->part2, nbs, h->part3, nbs, set_s0, st));
+        }
+        STP_HIP(hipMemsetAsync(h->p, 0, sizeof(float) * cols, st));
+        STP_HIP(hipMemsetAsync(&h->sc->beta, 0, sizeof(double), st));
+        STP_HIP(stp::launch_cgls_direction(h->sc, cols, h->z, h->p, h->part1, h->s, h->sp, st));
+        return STP_OK;
+    };
+
+    STP_HIP(hipMemsetAsync(x, 0, sizeof(double) * cols, st));
+    STP_HIP(hipMemsetAsync(h->sc, 0, sizeof(stp::SolverScalars), st));
+    STP_HIP(hipMemcpyAsync(h->r, b, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
+    int rc = start(1);
+    if (rc != STP_OK) return rc;
+    stp::SolverScalars hs;
+    std::memset(&hs, 0, sizeof hs);
+    int it = 0, replacements = 0;
+    double true_rel = 0.0;
+    std::vector<double> ps(nbs);
+    for (;;) {
+        for (; it < max_iter; ++it) {
+            STP_HIP(stp::spmv_f32(h->W, rows, h->rp, h->ci, h->val, h->p, h->q, h->part0, done, st));
+            STP_HIP(stp::launch_cgls_alpha(h->sc, h->part0, nbq, h->part1, nbc, lam, st));
+            STP_HIP(stp::launch_cgls_update(h->sc, cols, x, h->p, rows, h->r, h->q, st));
+            STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z,
+                                         h->part2, h->part3, done, st));
+            if (flex) {
+                rc = stp::lpc_apply(precond_handle, h->s, h->z, done, st);
+                if (rc != STP_OK) return rc;
+                STP_HIP(stp::launch_flex_dots(done, cols, h->z, h->s, h->sp, h->part4, h->part5, st));
+                STP_HIP(stp::launch_cgls_beta(h->sc, h->part4, nbc, h->part3, nbs, h->part5, nbc, tol, history, it,
+                                              st));
+            } else {
+                STP_HIP(stp::launch_cgls_beta(h->sc, h->part2, nbs, h->part3, nbs, nullptr, 0, tol, history, it, st));
+            }
+            STP_HIP(stp::launch_cgls_direction(h->sc, cols, h->z, h->p, h->part1, h->s, flex ? h->sp : nullptr, st));
+            if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
+                STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+                STP_HIP(hipStreamSynchronize(st));
+                if (hs.done) break;
+            }
+        }
+        STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+        STP_HIP(hipStreamSynchronize(st));
+        it = hs.iters;
+        // true residual in FP64
+        STP_HIP(stp::residual(h->W, rows, h->rp, h->ci, h->val, x, b, h->r, st));
+        STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z, h->part2,
+                                     h->part3, nullptr, st));
+        STP_HIP(hipMemcpyAsync(ps.data(), h->part3, sizeof(double) * nbs, hipMemcpyDeviceToHost, st));
+        STP_HIP(hipStreamSynchronize(st));
+        double ss = 0.0;
+        for (double v : ps) ss += v;
+        true_rel = hs.s0 > 0 ? std::sqrt(ss) / hs.s0 : 0.0;
+        if (hs.done != 1 || true_rel < tol || it >= max_iter || replacements >= 8) break;
+        ++replacements;
+        rc = start(0);
+        if (rc != STP_OK) return rc;
+        STP_HIP(hipMemsetAsync(&h->sc->done, 0, sizeof(int), st));
+    }
+    if (info) {
+        info->iterations = it;
+        info->converged = (hs.done == 1 || hs.s0 == 0) && true_rel < tol;
+        info->rel_residual = hs.rel;
+        info->true_rel_residual = true_rel;
+        info->replacements = replacements;
+    }
+    return STP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stp {
+
+// Device-resident solver scalars (one struct per solve).
+struct SolverScalars {
+    double alpha, beta, gamma, s0, rel;
+    int done;    // 0 running, 1 converged, 2 breakdown
+    int iters;
+};
+
+int spmv_lanes(double mean_nnz);
+int spmv_blocks(int rows, int W);
+hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
+                    double* part, const int* done, hipStream_t st);
+hipError_t spmv_f64(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, double* y,
+                    double alpha, const double* wv, double lam, double* part, const int* done, hipStream_t st);
+hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
+                           const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
+                           double* part_ss, const int* done, hipStream_t st);
+hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
+                    double* r, hipStream_t st);
+
+hipError_t rays_count(int64_t nrays, const double* pts, const int64_t* n, double h, int* counts, hipStream_t st);
+hipError_t rays_fill(int64_t nrays, const double* pts, const int64_t* n, double h, const int* rp, int* cols,
+                     float* vals, hipStream_t st);
+hipError_t exclusive_scan_i32(const int* in, int* out, int n, hipStream_t st);
+hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const int* ci, const float* val, int* trp,
+                         int* tci, float* tval, hipStream_t st);
+
+hipError_t launch_cgls_alpha(SolverScalars* sc, const double* pq, int nq, const double* pp, int np, double lam,
+                             hipStream_t st);
+hipError_t launch_cgls_beta(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
+                            const double* pflex, int nfl, double tol, double* history, int it, hipStream_t st);
+hipError_t launch_cgls_update(const SolverScalars* sc, int cols, double* x, const float* p, int rows, double* r,
+                              const float* q, hipStream_t st);
+hipError_t launch_cgls_direction(const SolverScalars* sc, int cols, const float* z, float* p, double* part_pp,
+                                 const float* s, float* s_prev, hipStream_t st);
+hipError_t launch_flex_dots(const int* done, int n, const float* z, const float* s, const float* s_prev,
+                            double* part_zs, double* part_fl, hipStream_t st);
+hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss, int set_s0,
+                            hipStream_t st);
+hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st);
+hipError_t launch_normal_diag(int n, const int* rp, const float* val, double lam, float* dinv, hipStream_t st);
+hipError_t launch_diag_inv(int n, const int* rp, const int* ci, const float* val, double* dinv, hipStream_t st);
+int cg_blocks(int n);
+hipError_t launch_dot_partial(const int* done, int n, const double* a, const double* b, double* part, hipStream_t st);
+hipError_t launch_cg_alpha(SolverScalars* sc, const double* part, int n, hipStream_t st);
+hipError_t launch_cg_update(const SolverScalars* sc, int n, double* x, double* r, const double* p, const double* ap,
+                            const double* dinv, double* z, double* prr, double* prz, hipStream_t st);
+hipError_t launch_cg_beta(SolverScalars* sc, const double* prr, const double* prz, int n, double tol, int it,
+                          double* history, hipStream_t st);
+hipError_t launch_cg_direction(const SolverScalars* sc, int n, const double* z, double* p, hipStream_t st);
+hipError_t launch_cg_init(SolverScalars* sc, const double* prr, const double* prz, int n, hipStream_t st);
+
+}  // namespace stp

@@ -1,0 +1,694 @@
+// Sparse kernels for the tomography solve on gfx950.
+//
+//   * CSR SpMV: W lanes of a wavefront cooperate on one row (W = 16/32/64
+//     chosen from the mean row length), lane-strided coalesced loads of the
+//     FP32 values and int32 columns, gather of x through L2, reduction with
+//     cross-lane shuffles.  Optional fused epilogues: y = a*Ax - lam*w and
+//     per-block FP64 partial sums of y*y / y*z for the solver's dot products.
+//   * Siddon straight-ray matrix: one thread per ray, exact path lengths in
+//     FP64 (no FP contraction, so they equal the CPU restatement bit for bit),
+//     rounded to FP32; rows sorted by column.
+//   * Transpose: radix sort of (col, row) keys — deterministic order.
+//   * Solver vector kernels with device-resident scalars: no host round trip
+//     per iteration; a `done` flag lets kernels of iterations queued after
+//     convergence return immediately.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <stdint.h>
+
+#include "sparse_kernels.h"
+
+namespace stp {
+
+// ------------------------------------------------------------------ SpMV
+template <typename T> __device__ __forceinline__ T shfl_xor_t(T v, int m) { return __shfl_xor(v, m); }
+
+template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART>
+__global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                    const float* __restrict__ val, const XT* __restrict__ x,
+                                                    YT* __restrict__ y, double alpha, const double* __restrict__ wv,
+                                                    double lam, double* __restrict__ part, const int* done) {
+    if (done && *done) return;
+    constexpr int RPB = 256 / W;
+    const int lane = threadIdx.x % W;
+    const int row = blockIdx.x * RPB + threadIdx.x / W;
+    AT acc = 0;
+    if (row < rows) {
+        const int b = rp[row], e = rp[row + 1];
+        for (int k = b + lane; k < e; k += W) acc += (AT)val[k] * (AT)x[ci[k]];
+    }
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) acc += shfl_xor_t(acc, m);
+    double out = 0.0;
+    if (row < rows && lane == 0) {
+        double v = alpha * (double)acc;
+        if (FUSE_W) v -= lam * wv[row];
+        y[row] = (YT)v;
+        out = (double)(YT)v;
+    }
+    if (PART) {
+        __shared__ double red[RPB];
+        if (lane == 0) red[threadIdx.x / W] = out * out;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int i = 0; i < RPB; ++i) s += red[i];
+            part[blockIdx.x] = s;
+        }
+    }
+}
+
+template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART>
+static hipError_t spmv_w(int rows, const int* rp, const int* ci, const float* val, const XT* x, YT* y, double alpha,
+                         const double* wv, double lam, double* part, const int* done, hipStream_t st) {
+    const int rpb = 256 / W;
+    const int nb = (rows + rpb - 1) / rpb;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL((spmv_kernel<W, XT, AT, YT, FUSE_W, PART>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, y,
+                       alpha, wv, lam, part, done);
+    return hipGetLastError();
+}
+
+int spmv_lanes(double mean_nnz) { return mean_nnz >= 48 ? 64 : (mean_nnz >= 20 ? 32 : 16); }
+
+int spmv_blocks(int rows, int W) { return (rows + 256 / W - 1) / (256 / W); }
+
+#define STP_SPMV_DISPATCH(XT, AT, YT, FW, PT)                                                                   \
+    switch (W) {                                                                                                \
+        case 64: return spmv_w<64, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
+        case 32: return spmv_w<32, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
+        default: return spmv_w<16, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
+    }
+
+hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
+                    double* part, const int* done, hipStream_t st) {
+    const double alpha = 1.0, lam = 0.0;
+    const double* wv = nullptr;
+    if (part) { STP_SPMV_DISPATCH(float, float, float, false, true) }
+    STP_SPMV_DISPATCH(float, float, float, false, false)
+}
+
+hipError_t spmv_f64(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, double* y,
+                    double alpha, const double* wv, double lam, double* part, const int* done, hipStream_t st) {
+    if (wv) {
+        if (part) { STP_SPMV_DISPATCH(double, double, double, true, true) }
+        STP_SPMV_DISPATCH(double, double, double, true, false)
+    }
+    if (part) { STP_SPMV_DISPATCH(double, double, double, false, true) }
+    STP_SPMV_DISPATCH(double, double, double, false, false)
+}
+
+// s = A^T r - lam*x with FP64 r and x, FP32 output s, z = dinv*s, partials of s*z and s*s
+template <int W>
+__global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const int* __restrict__ rp,
+                                                               const int* __restrict__ ci, const float* __restrict__ val,
+                                                               const double* __restrict__ r,
+                                                               const double* __restrict__ x, double lam,
+                                                               const float* __restrict__ dinv, float* __restrict__ s,
+                                                               float* __restrict__ z, double* __restrict__ part_sz,
+                                                               double* __restrict__ part_ss, const int* done) {
+    if (done && *done) return;
+    constexpr int RPB = 256 / W;
+    const int lane = threadIdx.x % W;
+    const int row = blockIdx.x * RPB + threadIdx.x / W;
+    double acc = 0.0;
+    if (row < rows) {
+        const int b = rp[row], e = rp[row + 1];
+        for (int k = b + lane; k < e; k += W) acc += (double)val[k] * r[ci[k]];
+    }
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
+    __shared__ double rsz[RPB], rss[RPB];
+    if (lane == 0) {
+        double sz = 0.0, ss = 0.0;
+        if (row < rows) {
+            const float sv = (float)(acc - lam * x[row]);
+            const float zv = dinv ? dinv[row] * sv : sv;
+            s[row] = sv;
+            z[row] = zv;
+            sz = (double)sv * (double)zv;
+            ss = (double)sv * (double)sv;
+        }
+        rsz[threadIdx.x / W] = sz;
+        rss[threadIdx.x / W] = ss;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < RPB; ++i) { a += rsz[i]; b += rss[i]; }
+        part_sz[blockIdx.x] = a;
+        part_ss[blockIdx.x] = b;
+    }
+}
+
+hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
+                           const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
+                           double* part_ss, const int* done, hipStream_t st) {
+    const int nb = spmv_blocks(rows, W);
+    if (nb == 0) return hipSuccess;
+    switch (W) {
+        case 64: hipLaunchKernelGGL(normal_residual_kernel<64>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
+                                    lam, dinv, s, z, part_sz, part_ss, done); break;
+        case 32: hipLaunchKernelGGL(normal_residual_kernel<32>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
+                                    lam, dinv, s, z, part_sz, part_ss, done); break;
+        default: hipLaunchKernelGGL(normal_residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
+                                    lam, dinv, s, z, part_sz, part_ss, done); break;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ reductions / scalars
+// Sum `n` partials into *out with one 256-thread block (fixed order: deterministic).
+__device__ double block_sum(const double* part, int n) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// CGLS step length: alpha = gamma / (|q|^2 + lam |p|^2)
+__global__ void cgls_alpha_kernel(SolverScalars* sc, const double* pq, int nq, const double* pp, int np, double lam) {
+    if (sc->done) return;
+    const double qq = block_sum(pq, nq);
+    const double ppv = block_sum(pp, np);
+    if (threadIdx.x == 0) {
+        const double del = qq + lam * ppv;
+        if (!(del > 0.0)) { sc->done = 2; sc->alpha = 0.0; return; }
+        sc->alpha = sc->gamma / del;
+    }
+}
+
+// after s = A^T r - lam x: record |s|/s0, test convergence, beta = gamma'/gamma
+// (flexible variant: gamma' = z.s, beta = z.(s - s_prev)/gamma)
+__global__ void cgls_beta_kernel(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
+                                 const double* pflex, int nfl, double tol, double* history, int it) {
+    if (sc->done) return;
+    const double sz = block_sum(psz, nsz);
+    const double ss = block_sum(pss, nss);
+    const double fl = pflex ? block_sum(pflex, nfl) : 0.0;
+    if (threadIdx.x == 0) {
+        const double rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
+        sc->rel = rel;
+        sc->iters = it + 1;
+        if (history) history[it] = rel;
+        if (rel < tol) { sc->done = 1; return; }
+        sc->beta = (pflex ? fl : sz) / sc->gamma;
+        sc->gamma = sz;
+    }
+}
+
+// x += alpha p (cols), r -= alpha q (rows)
+__global__ void cgls_update_kernel(const SolverScalars* sc, int cols, double* x, const float* p, int rows, double* r,
+                                   const float* q) {
+    if (sc->done) return;
+    const double a = sc->alpha;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cols) x[i] += a * (double)p[i];
+    if (i < rows) r[i] -= a * (double)q[i];
+}
+
+// p = z + beta p, partial |p|^2 ; optional s_prev <- s (flexible variant)
+__global__ void cgls_direction_kernel(const SolverScalars* sc, int cols, const float* z, float* p, double* part_pp,
+                                      const float* s, float* s_prev) {
+    if (sc->done) return;
+    const double b = sc->beta;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < cols) {
+        const float pv = (float)((double)z[i] + b * (double)p[i]);
+        p[i] = pv;
+        v = (double)pv * (double)pv;
+        if (s_prev) s_prev[i] = s[i];
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part_pp[blockIdx.x] = red[0];
+}
+
+// partials of z.s and z.(s - s_prev) (flexible variant)
+__global__ void flex_dots_kernel(const int* done, int n, const float* z, const float* s, const float* s_prev,
+                                 double* part_zs, double* part_fl) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < n) {
+        a = (double)z[i] * (double)s[i];
+        b = (double)z[i] * ((double)s[i] - (double)s_prev[i]);
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part_zs[blockIdx.x] = red[0][0]; part_fl[blockIdx.x] = red[1][0]; }
+}
+
+// gamma <- sum(psz), s0 <- sqrt(sum(pss)) (initialisation) or only gamma (restart)
+__global__ void cgls_init_kernel(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
+                                 int set_s0) {
+    const double sz = block_sum(psz, nsz);
+    const double ss = block_sum(pss, nss);
+    if (threadIdx.x == 0) {
+        sc->gamma = sz;
+        if (set_s0) { sc->s0 = sqrt(ss); sc->done = sc->s0 > 0.0 ? 0 : 1; sc->iters = 0; }
+        sc->rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
+    }
+}
+
+__global__ void copy_f32_kernel(int n, const float* a, float* b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// r = b - A x  (FP64 x, FP64 accumulation)
+template <int W>
+__global__ void __launch_bounds__(256) residual_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                       const float* __restrict__ val, const double* __restrict__ x,
+                                                       const double* __restrict__ b, double* __restrict__ r) {
+    constexpr int RPB = 256 / W;
+    const int lane = threadIdx.x % W;
+    const int row = blockIdx.x * RPB + threadIdx.x / W;
+    double acc = 0.0;
+    if (row < rows) {
+        const int s = rp[row], e = rp[row + 1];
+        for (int k = s + lane; k < e; k += W) acc += (double)val[k] * x[ci[k]];
+    }
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
+    if (row < rows && lane == 0) r[row] = b[row] - acc;
+}
+
+hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
+                    double* r, hipStream_t st) {
+    const int nb = spmv_blocks(rows, W);
+    if (nb == 0) return hipSuccess;
+    switch (W) {
+        case 64: hipLaunchKernelGGL(residual_kernel<64>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
+        case 32: hipLaunchKernelGGL(residual_kernel<32>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
+        default: hipLaunchKernelGGL(residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
+    }
+    return hipGetLastError();
+}
+
+// Jacobi for A^T A + lam I: dinv_j = 1 / (sum_i a_ij^2 + lam), from the rows of A^T
+__global__ void normal_diag_kernel(int n, const int* rp, const float* val, double lam, float* dinv) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double s = lam;
+    for (int k = rp[j]; k < rp[j + 1]; ++k) s += (double)val[k] * (double)val[k];
+    dinv[j] = s > 0.0 ? (float)(1.0 / s) : 1.0f;
+}
+
+// Jacobi for a square A (reference jacobi_preconditioned_cg): first diagonal
+// entry of each row with |a| > 1e-30, else 1
+__global__ void diag_inv_kernel(int n, const int* rp, const int* ci, const float* val, double* dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 1.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) {
+            if (fabs((double)val[k]) > 1e-30) d = 1.0 / (double)val[k];
+            break;
+        }
+    dinv[i] = d;
+}
+
+// ------------------------------------------------------------------ square CG (reference semantics)
+// pap = p.Ap partials are produced by spmv with a custom epilogue: reuse dot kernel
+__global__ void dot_partial_kernel(const int* done, int n, const double* a, const double* b, double* part) {
+    if (done && *done) return;
+    double v = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += a[i] * b[i];
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void cg_alpha_kernel(SolverScalars* sc, const double* part, int n) {
+    if (sc->done) return;
+    const double pap = block_sum(part, n);
+    if (threadIdx.x == 0) {
+        if (fabs(pap) < 1e-30) { sc->done = 2; return; }
+        sc->alpha = sc->gamma / pap;
+    }
+}
+
+// x += a p; r -= a Ap; z = dinv r (or r); partials r.r and r.z
+__global__ void cg_update_kernel(const SolverScalars* sc, int n, double* x, double* r, const double* p,
+                                 const double* ap, const double* dinv, double* z, double* part_rr, double* part_rz) {
+    if (sc->done) return;
+    const double a = sc->alpha;
+    double rr = 0.0, rz = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * ap[i];
+        r[i] = ri;
+        const double zi = dinv ? dinv[i] * ri : ri;
+        if (dinv) z[i] = zi;
+        rr += ri * ri;
+        rz += ri * zi;
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = rr;
+    red[1][threadIdx.x] = rz;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part_rr[blockIdx.x] = red[0][0]; part_rz[blockIdx.x] = red[1][0]; }
+}
+
+// convergence test on |r|/|b| (reference: CG and PCG both test before the p update)
+__global__ void cg_beta_kernel(SolverScalars* sc, const double* prr, const double* prz, int n, double tol, int it,
+                               double* history) {
+    if (sc->done) return;
+    const double rr = block_sum(prr, n);
+    const double rz = block_sum(prz, n);
+    if (threadIdx.x == 0) {
+        sc->iters = it + 1;
+        sc->rel = sqrt(rr) / sc->s0;
+        if (history) history[it] = sc->rel;
+        if (sqrt(rr) / sc->s0 < tol) { sc->done = 1; return; }
+        sc->beta = rz / sc->gamma;
+        sc->gamma = rz;
+    }
+}
+
+__global__ void cg_direction_kernel(const SolverScalars* sc, int n, const double* z, double* p) {
+    if (sc->done) return;
+    const double b = sc->beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + b * p[i];
+}
+
+__global__ void cg_init_kernel(SolverScalars* sc, const double* prr, const double* prz, int n) {
+    const double rr = block_sum(prr, n);
+    const double rz = block_sum(prz, n);
+    if (threadIdx.x == 0) {
+        sc->s0 = sqrt(rr);
+        sc->gamma = rz;
+        sc->iters = 0;
+        sc->rel = sc->s0 < 1e-30 ? 0.0 : 1.0;
+        sc->done = sc->s0 < 1e-30 ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------------------ Siddon rays
+#pragma clang fp contract(off)
+struct RayWalk {
+    double p0[3], d[3], len, tnext[3];
+    int64_t knext[3], kend[3], kdir[3];
+    double h, t0;
+    int64_t n[3];
+    bool fin;
+};
+
+__device__ __forceinline__ bool axis_exhausted(int64_t k, int64_t kend, int64_t kdir) {
+    return kdir > 0 ? k > kend : k < kend;
+}
+
+__device__ void ray_init(RayWalk& w, const double* pt, const int64_t* n, double h) {
+    double l2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        w.p0[a] = pt[a];
+        w.d[a] = pt[3 + a] - pt[a];
+        l2 += w.d[a] * w.d[a];
+        w.n[a] = n[a];
+    }
+    w.len = sqrt(l2);
+    w.h = h;
+    w.t0 = 0.0;
+    w.fin = w.len == 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double p1 = pt[3 + a];
+        if (w.d[a] > 0) {
+            w.kdir[a] = 1;
+            w.knext[a] = (int64_t)floor(w.p0[a] / h) + 1;
+            w.kend[a] = (int64_t)ceil(p1 / h) - 1;
+        } else if (w.d[a] < 0) {
+            w.kdir[a] = -1;
+            w.knext[a] = (int64_t)ceil(w.p0[a] / h) - 1;
+            w.kend[a] = (int64_t)floor(p1 / h) + 1;
+        } else {
+            w.kdir[a] = 0; w.knext[a] = 0; w.kend[a] = 0;
+        }
+        w.tnext[a] = w.d[a] != 0 ? ((double)w.knext[a] * h - w.p0[a]) / w.d[a] : INFINITY;
+        if (w.d[a] != 0 && axis_exhausted(w.knext[a], w.kend[a], w.kdir[a])) w.tnext[a] = INFINITY;
+    }
+}
+
+// next segment; returns false when the walk is over
+__device__ bool ray_next(RayWalk& w, int64_t* cell, double* seg) {
+    while (!w.fin) {
+        int a = 0;
+        if (w.tnext[1] < w.tnext[a]) a = 1;
+        if (w.tnext[2] < w.tnext[a]) a = 2;
+        const double t1 = w.tnext[a] < 1.0 ? w.tnext[a] : 1.0;
+        const double t0 = w.t0;
+        bool emit = false;
+        if (t1 > t0) {
+            const double tm = 0.5 * (t0 + t1);
+            int64_t ix = (int64_t)floor((w.p0[0] + tm * w.d[0]) / w.h);
+            int64_t iy = (int64_t)floor((w.p0[1] + tm * w.d[1]) / w.h);
+            int64_t iz = (int64_t)floor((w.p0[2] + tm * w.d[2]) / w.h);
+            ix = ix < 0 ? 0 : (ix >= w.n[0] ? w.n[0] - 1 : ix);
+            iy = iy < 0 ? 0 : (iy >= w.n[1] ? w.n[1] - 1 : iy);
+            iz = iz < 0 ? 0 : (iz >= w.n[2] ? w.n[2] - 1 : iz);
+            *cell = (ix * w.n[1] + iy) * w.n[2] + iz;
+            *seg = (t1 - t0) * w.len;
+            emit = true;
+        }
+        if (t1 >= 1.0) {
+            w.fin = true;
+        } else {
+            w.t0 = t1;
+            w.knext[a] += w.kdir[a];
+            w.tnext[a] = axis_exhausted(w.knext[a], w.kend[a], w.kdir[a])
+                             ? INFINITY
+                             : ((double)w.knext[a] * w.h - w.p0[a]) / w.d[a];
+        }
+        if (emit) return true;
+    }
+    return false;
+}
+#pragma clang fp contract(on)
+
+__global__ void rays_count_kernel(int64_t nrays, const double* pts, int64_t n0, int64_t n1, int64_t n2, double h,
+                                  int* counts) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrays) return;
+    const int64_t n[3] = {n0, n1, n2};
+    RayWalk w;
+    ray_init(w, pts + 6 * r, n, h);
+    int c = 0;
+    int64_t cell;
+    double seg;
+    while (ray_next(w, &cell, &seg)) ++c;
+    counts[r] = c;
+}
+
+// fill the row in traversal order, then insertion-sort it by column
+__global__ void rays_fill_kernel(int64_t nrays, const double* pts, int64_t n0, int64_t n1, int64_t n2, double h,
+                                 const int* rp, int* cols, float* vals) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrays) return;
+    const int64_t n[3] = {n0, n1, n2};
+    RayWalk w;
+    ray_init(w, pts + 6 * r, n, h);
+    const int b = rp[r], e = rp[r + 1];
+    int k = b;
+    int64_t cell;
+    double seg;
+    while (k < e && ray_next(w, &cell, &seg)) {
+        const int c = (int)cell;
+        const float v = (float)seg;
+        int j = k;
+        while (j > b && cols[j - 1] > c) { cols[j] = cols[j - 1]; vals[j] = vals[j - 1]; --j; }
+        cols[j] = c;
+        vals[j] = v;
+        ++k;
+    }
+}
+
+hipError_t rays_count(int64_t nrays, const double* pts, const int64_t* n, double h, int* counts, hipStream_t st) {
+    const unsigned nb = (unsigned)((nrays + 127) / 128);
+    if (nb) hipLaunchKernelGGL(rays_count_kernel, dim3(nb), dim3(128), 0, st, nrays, pts, n[0], n[1], n[2], h, counts);
+    return hipGetLastError();
+}
+
+hipError_t rays_fill(int64_t nrays, const double* pts, const int64_t* n, double h, const int* rp, int* cols,
+                     float* vals, hipStream_t st) {
+    const unsigned nb = (unsigned)((nrays + 127) / 128);
+    if (nb)
+        hipLaunchKernelGGL(rays_fill_kernel, dim3(nb), dim3(128), 0, st, nrays, pts, n[0], n[1], n[2], h, rp, cols,
+                           vals);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ transpose
+__global__ void t_keys_kernel(int rows, const int* rp, const int* ci, uint64_t* keys, int* counts) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        keys[k] = ((uint64_t)(uint32_t)ci[k] << 32) | (uint32_t)r;
+        atomicAdd(&counts[ci[k]], 1);
+    }
+}
+
+__global__ void t_cols_kernel(int64_t nnz, const uint64_t* keys, int* tci) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) tci[i] = (int)(keys[i] & 0xffffffffu);
+}
+
+hipError_t exclusive_scan_i32(const int* in, int* out, int n, hipStream_t st) {
+    size_t tb = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, st);
+    if (e != hipSuccess) return e;
+    void* tmp = nullptr;
+    if ((e = hipMalloc(&tmp, tb ? tb : 1)) != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(tmp);
+    return e;
+}
+
+// A (rows x cols) -> A^T (cols x rows), entries of a column ordered by row
+hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const int* ci, const float* val, int* trp,
+                         int* tci, float* tval, hipStream_t st) {
+    hipError_t e = hipSuccess;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    float* v1 = nullptr;
+    int* counts = nullptr;
+    void* tmp = nullptr;
+    size_t tb = 0;
+    do {
+        if ((e = hipMalloc(&counts, sizeof(int) * ((size_t)cols + 1))) != hipSuccess) break;
+        if ((e = hipMemsetAsync(counts, 0, sizeof(int) * ((size_t)cols + 1), st)) != hipSuccess) break;
+        if (nnz > 0) {
+            if ((e = hipMalloc(&k0, sizeof(uint64_t) * nnz)) != hipSuccess) break;
+            if ((e = hipMalloc(&k1, sizeof(uint64_t) * nnz)) != hipSuccess) break;
+            if ((e = hipMalloc(&v1, sizeof(float) * nnz)) != hipSuccess) break;
+            hipLaunchKernelGGL(t_keys_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, rows, rp, ci, k0, counts);
+            if ((e = hipGetLastError()) != hipSuccess) break;
+            if ((e = hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, val, v1, (int)nnz, 0, 64, st)) != hipSuccess)
+                break;
+            if ((e = hipMalloc(&tmp, tb ? tb : 1)) != hipSuccess) break;
+            if ((e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, val, v1, (int)nnz, 0, 64, st)) != hipSuccess)
+                break;
+            hipLaunchKernelGGL(t_cols_kernel, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, nnz, k1, tci);
+            if ((e = hipGetLastError()) != hipSuccess) break;
+            if ((e = hipMemcpyAsync(tval, v1, sizeof(float) * nnz, hipMemcpyDeviceToDevice, st)) != hipSuccess) break;
+        }
+        if ((e = exclusive_scan_i32(counts, trp, cols + 1, st)) != hipSuccess) break;
+    } while (0);
+    hipStreamSynchronize(st);
+    hipFree(tmp); hipFree(k0); hipFree(k1); hipFree(v1); hipFree(counts);
+    return e;
+}
+
+// ------------------------------------------------------------------ launch helpers
+static unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+hipError_t launch_cgls_alpha(SolverScalars* sc, const double* pq, int nq, const double* pp, int np, double lam,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(cgls_alpha_kernel, dim3(1), dim3(256), 0, st, sc, pq, nq, pp, np, lam);
+    return hipGetLastError();
+}
+hipError_t launch_cgls_beta(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
+                            const double* pflex, int nfl, double tol, double* history, int it, hipStream_t st) {
+    hipLaunchKernelGGL(cgls_beta_kernel, dim3(1), dim3(256), 0, st, sc, psz, nsz, pss, nss, pflex, nfl, tol, history,
+                       it);
+    return hipGetLastError();
+}
+hipError_t launch_cgls_update(const SolverScalars* sc, int cols, double* x, const float* p, int rows, double* r,
+                              const float* q, hipStream_t st) {
+    hipLaunchKernelGGL(cgls_update_kernel, dim3(nb256(cols > rows ? cols : rows)), dim3(256), 0, st, sc, cols, x, p,
+                       rows, r, q);
+    return hipGetLastError();
+}
+hipError_t launch_cgls_direction(const SolverScalars* sc, int cols, const float* z, float* p, double* part_pp,
+                                 const float* s, float* s_prev, hipStream_t st) {
+    hipLaunchKernelGGL(cgls_direction_kernel, dim3(nb256(cols)), dim3(256), 0, st, sc, cols, z, p, part_pp, s, s_prev);
+    return hipGetLastError();
+}
+hipError_t launch_flex_dots(const int* done, int n, const float* z, const float* s, const float* s_prev,
+                            double* part_zs, double* part_fl, hipStream_t st) {
+    hipLaunchKernelGGL(flex_dots_kernel, dim3(nb256(n)), dim3(256), 0, st, done, n, z, s, s_prev, part_zs, part_fl);
+    return hipGetLastError();
+}
+hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss, int set_s0,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(cgls_init_kernel, dim3(1), dim3(256), 0, st, sc, psz, nsz, pss, nss, set_s0);
+    return hipGetLastError();
+}
+hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st) {
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(nb256(n)), dim3(256), 0, st, n, a, b);
+    return hipGetLastError();
+}
+hipError_t launch_normal_diag(int n, const int* rp, const float* val, double lam, float* dinv, hipStream_t st) {
+    hipLaunchKernelGGL(normal_diag_kernel, dim3(nb256(n)), dim3(256), 0, st, n, rp, val, lam, dinv);
+    return hipGetLastError();
+}
+hipError_t launch_diag_inv(int n, const int* rp, const int* ci, const float* val, double* dinv, hipStream_t st) {
+    hipLaunchKernelGGL(diag_inv_kernel, dim3(nb256(n)), dim3(256), 0, st, n, rp, ci, val, dinv);
+    return hipGetLastError();
+}
+int cg_blocks(int n) {
+    int b = (n + 255) / 256;
+    return b > 1024 ? 1024 : (b < 1 ? 1 : b);
+}
+hipError_t launch_dot_partial(const int* done, int n, const double* a, const double* b, double* part, hipStream_t st) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(cg_blocks(n)), dim3(256), 0, st, done, n, a, b, part);
+    return hipGetLastError();
+}
+hipError_t launch_cg_alpha(SolverScalars* sc, const double* part, int n, hipStream_t st) {
+    hipLaunchKernelGGL(cg_alpha_kernel, dim3(1), dim3(256), 0, st, sc, part, n);
+    return hipGetLastError();
+}
+hipError_t launch_cg_update(const SolverScalars* sc, int n, double* x, double* r, const double* p, const double* ap,
+                            const double* dinv, double* z, double* prr, double* prz, hipStream_t st) {
+    hipLaunchKernelGGL(cg_update_kernel, dim3(cg_blocks(n)), dim3(256), 0, st, sc, n, x, r, p, ap, dinv, z, prr, prz);
+    return hipGetLastError();
+}
+hipError_t launch_cg_beta(SolverScalars* sc, const double* prr, const double* prz, int n, double tol, int it,
+                          double* history, hipStream_t st) {
+    hipLaunchKernelGGL(cg_beta_kernel, dim3(1), dim3(256), 0, st, sc, prr, prz, n, tol, it, history);
+    return hipGetLastError();
+}
+hipError_t launch_cg_direction(const SolverScalars* sc, int n, const double* z, double* p, hipStream_t st) {
+    hipLaunchKernelGGL(cg_direction_kernel, dim3(cg_blocks(n)), dim3(256), 0, st, sc, n, z, p);
+    return hipGetLastError();
+}
+hipError_t launch_cg_init(SolverScalars* sc, const double* prr, const double* prz, int n, hipStream_t st) {
+    hipLaunchKernelGGL(cg_init_kernel, dim3(1), dim3(256), 0, st, sc, prr, prz, n);
+    return hipGetLastError();
+}
+
+}  // namespace stp

@@ -127,6 +127,56 @@ int stp_clamp_field_f64(double* data_dev, int64_t n, double lo, double hi, void*
 int stp_repair_field_nan_f64(double* data_dev, int64_t count, int64_t nx, int64_t ny, int64_t nz,
                              int32_t halo_width, double replace_value, void* stream);
 
+
+/* ---------------------------------------------------------------- tomography
+ * Device CSR of the straight-ray matrix A (FP32 values, int32 columns and row
+ * offsets) plus an explicit A^T built on the device (deterministic order).
+ * Replaces CSRMatrix's storage and SpMV (src/cpp/sparse/csr_matrix.h:17-71,
+ * csr_matrix.cpp:118-166) and SparseOps' solvers (sparse_ops.cpp:8-113). */
+typedef struct stp_csr stp_csr;
+
+typedef struct {
+    int32_t iterations;         /* iterations run */
+    int32_t converged;          /* 1 if the (true) relative residual < tol */
+    double rel_residual;        /* recursive residual at exit */
+    double true_rel_residual;   /* residual recomputed in FP64 at exit */
+    int32_t replacements;       /* FP64 residual replacements (refinement restarts) */
+} stp_solve_info;
+
+int stp_csr_create(int64_t rows, int64_t cols, int64_t nnz, const int32_t* row_ptr_dev, const int32_t* col_dev,
+                   const float* val_dev, stp_csr** out, void* stream);
+/* Exact straight-ray (Siddon) matrix: ray r runs from pts[6r..6r+2] to
+ * pts[6r+3..6r+5] (device FP64) through an n[0] x n[1] x n[2] grid of cubic
+ * cells of size h (origin 0); entry = path length inside the cell (FP64
+ * arithmetic rounded to FP32), columns ix*n1*n2 + iy*n2 + iz sorted per row.
+ * New: the reference samples rays matrix-free (inversion.py:329-383). */
+int stp_csr_from_rays(int64_t nrays, const int64_t* n_host, double h, const double* pts_dev, stp_csr** out,
+                      void* stream);
+int stp_csr_destroy(stp_csr* A);
+int stp_csr_shape(const stp_csr* A, int64_t* rows, int64_t* cols, int64_t* nnz);
+int stp_csr_export(const stp_csr* A, int32_t transpose, int32_t* row_ptr_dev, int32_t* col_dev, float* val_dev,
+                   void* stream);
+/* y = A x (transpose = 0) or A^T x; FP32 accumulation */
+int stp_csr_spmv(const stp_csr* A, int32_t transpose, const float* x_dev, float* y_dev, void* stream);
+/* same with FP64 x, FP64 accumulation and FP64 y (values stay FP32) */
+int stp_csr_spmv_f64(const stp_csr* A, int32_t transpose, const double* x_dev, double* y_dev, void* stream);
+
+/* SparseOps::conjugate_gradient / jacobi_preconditioned_cg on a square A:
+ * x0 = 0, stop when |r|/|b| < tol (sparse_ops.cpp:39, 97-98), breakdown when
+ * |p.Ap| < 1e-30; Jacobi = first diagonal entry with |a| > 1e-30, else 1.
+ * FP64 vectors and dots, FP32 matrix values.  history_dev (optional) gets
+ * |r|/|b| after each iteration. */
+int stp_cg(const stp_csr* A, const double* b_dev, int32_t max_iter, double tol, int32_t jacobi, double* x_dev,
+           double* history_dev, stp_solve_info* info, void* stream);
+
+/* CGLS on (A^T A + lambda I) x = A^T b, x0 = 0, stop when
+ * |A^T r - lambda x| / |A^T b| < tol.  precond: 0 none, 1 Jacobi
+ * (1/(sum_i a_ij^2 + lambda)), 2 learned network (precond_handle = stp_lpc*,
+ * flexible CG).  FP32 SpMV, FP64 x, r and dots, FP64 residual replacement.
+ * New: the reference has CG on square systems only. */
+int stp_cgls(const stp_csr* A, const double* b_dev, double lambda, double tol, int32_t max_iter, int32_t precond,
+             void* precond_handle, double* x_dev, double* history_dev, stp_solve_info* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
