@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -41,7 +42,7 @@ int fd_weights(int order, double* a) {
 
 struct stp_fd_plan {
     stp_fd_config cfg;
-    int R = 0, VW = 1, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
+    int R = 0, VW = 1, variant = 0, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
     int nxh = 0, nyh = 0, nzh = 0;
     int64_t ncell = 0;
     float cx[5] = {0}, cy[5] = {0}, cz[5] = {0};
@@ -113,14 +114,19 @@ int stp_fd_plan_create(const stp_fd_config* c, stp_fd_plan** out) {
         return fail(STP_ERR_INVALID, "stp_fd_plan_create: spacings and dt must be positive");
     if (c->nshots < 1) return fail(STP_ERR_INVALID, "stp_fd_plan_create: nshots < 1");
     const int64_t nxh = c->nx + 2 * c->halo_width, nyh = c->ny + 2 * c->halo_width, nzh = c->nz + 2 * c->halo_width;
-    if (nxh > (1 << 30) || nyh > (1 << 30) || nzh > (1 << 30) || nyh * nzh > ((int64_t)1 << 40))
-        return fail(STP_ERR_INVALID, "stp_fd_plan_create: grid too large");
+    if (nxh > (1 << 30) || nyh > (1 << 30) || nzh > (1 << 30) || nxh * nyh * nzh >= ((int64_t)1 << 32))
+        return fail(STP_ERR_INVALID, "stp_fd_plan_create: a shot's field must have fewer than 2^32 cells");
     stp_fd_plan* p = new (std::nothrow) stp_fd_plan();
     if (!p) return fail(STP_ERR_ALLOC, "stp_fd_plan_create: out of host memory");
     p->cfg = *c;
     p->R = R;
     p->VW = (c->halo_width % 4 == 0 && c->nz % 4 == 0) ? 4 : 1;
-    stp::fd_tiling(R, p->VW, &p->tz, &p->ty, &p->lx);
+    // tile-shape override for tuning: STP_FD_TILE=<variant index>
+    if (const char* ev = std::getenv("STP_FD_TILE")) {
+        const int v = std::atoi(ev);
+        if (v >= 0 && v < stp::fd_num_variants()) p->variant = v;
+    }
+    stp::fd_tiling(R, p->VW, p->variant, &p->tz, &p->ty, &p->lx);
     p->nxh = (int)nxh; p->nyh = (int)nyh; p->nzh = (int)nzh;
     p->ncell = nxh * nyh * nzh;
     p->ntz = (int)((c->nz + p->tz - 1) / p->tz);
@@ -296,7 +302,7 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
         a.t_col = s;
         a.w_next = (inject && s + 1 < nsteps && t + 1 < nw) ? wavelet + (t + 1) : nullptr;
         a.step_status = status ? status + s : nullptr;
-        STP_HIP(stp::fd_launch_step(a, p->R, p->VW, S(st)));
+        STP_HIP(stp::fd_launch_step(a, p->R, p->VW, p->variant, S(st)));
         std::swap(u, v);
     }
     return STP_OK;

@@ -36,8 +36,9 @@ struct FdStepArgs {
     int* step_status;         // OR-ed with STP_STEP_NAN_DETECTED_AND_REPAIRED
 };
 
-void fd_tiling(int R, int VW, int* tz, int* ty, int* lx);
-hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, hipStream_t st);
+int fd_num_variants();
+void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx);
+hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipStream_t st);
 hipError_t fd_launch_inject(float* u, int64_t ncell, const int64_t* src_idx, const float* w, int64_t w_stride,
                             int nshots, hipStream_t st);
 hipError_t fd_launch_coef(const float* vel, float* coef, int64_t n, float dt2, float damp, hipStream_t st);

@@ -107,7 +107,6 @@ fd_step_kernel(const FdStepArgs a) {
     const int shot = rem;
 
     const int hw = a.hw, nyh = a.nyh, nzh = a.nzh, nxh = a.nxh;
-    const int64_t plane = (int64_t)nyh * nzh;
     const int zb = hw + zt * TZ, yb = hw + yt * TY;
     const int x0 = hw + xc * LX;
     const int x1 = min(x0 + LX, nxh - hw);
@@ -116,21 +115,28 @@ fd_step_kernel(const FdStepArgs a) {
     float* __restrict__ P = a.uprev + (int64_t)shot * a.ncell;
     const float* __restrict__ C = a.coef;
 
+    // Offsets are 32-bit element offsets from the per-shot base (a shot's
+    // field is < 2^32 elements, checked at plan creation).  Cells outside the
+    // grid load from a clamped in-bounds address and are never stored, which
+    // keeps the loads branch-free.
+    const uint32_t plane = (uint32_t)nyh * (uint32_t)nzh;
+    const uint32_t safe = (uint32_t)yb * nzh + (uint32_t)zb;
+
     // own cells
     const int z = zb + tz * VW;
     const bool zvalid = z < nzh - hw;
     bool rvalid[TYR];
-    int64_t roff[TYR];   // offset of own vector within a plane
+    uint32_t roff[TYR];   // offset of own vector within a plane
 #pragma unroll
     for (int r = 0; r < TYR; ++r) {
         const int y = yb + ty * TYR + r;
         rvalid[r] = zvalid && (y < nyh - hw);
-        roff[r] = rvalid[r] ? (int64_t)y * nzh + z : 0;
+        roff[r] = rvalid[r] ? (uint32_t)y * nzh + (uint32_t)z : safe;
     }
 
-    // halo items: LDS slot and in-plane global offset (or -1 = zero fill)
+    // halo items: LDS slot (-1 = none) and in-plane offset
     int hslot[NH];
-    int64_t hoff[NH];
+    uint32_t hoff[NH];
 #pragma unroll
     for (int m = 0; m < NH; ++m) {
         const int i = tid + m * NT;
@@ -152,7 +158,7 @@ fd_step_kernel(const FdStepArgs a) {
         const int gy = yb - R + ly;
         const int gz = zb + (lz - HV) * VW;
         const bool inb = use && gy >= 0 && gy < nyh && gz >= 0 && gz + VW <= nzh;
-        hoff[m] = inb ? (int64_t)gy * nzh + gz : -1;
+        hoff[m] = inb ? (uint32_t)gy * nzh + (uint32_t)gz : safe;
     }
 
     // sponge: Dyz per own cell (constant along x)
@@ -170,7 +176,7 @@ fd_step_kernel(const FdStepArgs a) {
         }
     }
 
-    auto ldv = [&](const float* base, int64_t off) -> vec {
+    auto ldv = [](const float* base, uint32_t off) -> vec {
         return *reinterpret_cast<const vec*>(base + off);
     };
 
@@ -178,23 +184,23 @@ fd_step_kernel(const FdStepArgs a) {
     vec q[2 * R + 1][TYR];
 #pragma unroll
     for (int k = 1; k <= 2 * R; ++k) {
-        const int64_t pb = (int64_t)(x0 - R + k - 1) * plane;
+        const uint32_t pb = (uint32_t)(x0 - R + k - 1) * plane;
 #pragma unroll
-        for (int r = 0; r < TYR; ++r) q[k][r] = rvalid[r] ? ldv(U, pb + roff[r]) : V::zero();
+        for (int r = 0; r < TYR; ++r) q[k][r] = ldv(U, pb + roff[r]);
     }
     // prefetch for plane x0
     vec qn[TYR], pn[TYR], cn[TYR], hn[NH];
     {
-        const int64_t pb = (int64_t)x0 * plane;
-        const int64_t pf = (int64_t)(x0 + R) * plane;
+        const uint32_t pb = (uint32_t)x0 * plane;
+        const uint32_t pf = (uint32_t)(x0 + R) * plane;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
-            qn[r] = rvalid[r] ? ldv(U, pf + roff[r]) : V::zero();
-            pn[r] = rvalid[r] ? ldv(P, pb + roff[r]) : V::zero();
-            cn[r] = rvalid[r] ? ldv(C, pb + roff[r]) : V::zero();
+            qn[r] = ldv(U, pf + roff[r]);
+            pn[r] = ldv(P, pb + roff[r]);
+            cn[r] = ldv(C, pb + roff[r]);
         }
 #pragma unroll
-        for (int m = 0; m < NH; ++m) hn[m] = hoff[m] >= 0 ? ldv(U, pb + hoff[m]) : V::zero();
+        for (int m = 0; m < NH; ++m) hn[m] = ldv(U, pb + hoff[m]);
     }
 
     const float amp = a.amp;
@@ -220,23 +226,26 @@ fd_step_kernel(const FdStepArgs a) {
         for (int m = 0; m < NH; ++m)
             if (hslot[m] >= 0) (&L[0][0])[hslot[m]] = hn[m];
 
-        // prefetch plane x+1
-        if (x + 1 < x1) {
-            const int64_t pb = (int64_t)(x + 1) * plane;
-            const int64_t pf = (int64_t)(x + 1 + R) * plane;
+        // prefetch plane x+1 (the last iteration re-reads plane x: in bounds, unused)
+        {
+            const int xn = x + 1 < x1 ? x + 1 : x;
+            const uint32_t pb = (uint32_t)xn * plane;
+            const uint32_t pf = (uint32_t)(xn + R) * plane;
 #pragma unroll
             for (int r = 0; r < TYR; ++r) {
-                qn[r] = rvalid[r] ? ldv(U, pf + roff[r]) : V::zero();
-                pn[r] = rvalid[r] ? ldv(P, pb + roff[r]) : V::zero();
-                cn[r] = rvalid[r] ? ldv(C, pb + roff[r]) : V::zero();
+                qn[r] = ldv(U, pf + roff[r]);
+                pn[r] = ldv(P, pb + roff[r]);
+                cn[r] = ldv(C, pb + roff[r]);
             }
 #pragma unroll
-            for (int m = 0; m < NH; ++m) hn[m] = hoff[m] >= 0 ? ldv(U, pb + hoff[m]) : V::zero();
+            for (int m = 0; m < NH; ++m) hn[m] = ldv(U, pb + hoff[m]);
         }
         __syncthreads();
 
         const float dxs = sponge_of(sponge_dist(x, nxh, hw, sw), sw, sg);
-        const int64_t pb = (int64_t)x * plane;
+        const uint32_t pb = (uint32_t)x * plane;
+        vec res[TYR];
+        bool bad = false;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
             const int lr = R + ty * TYR + r;
@@ -257,7 +266,6 @@ fd_step_kernel(const FdStepArgs a) {
                 else if (rr >= 0 && rr < TYR) yc[R + k] = q[R][rr];
                 else yc[R + k] = L[lr + k][HV + tz];
             }
-            vec out;
 #pragma unroll
             for (int j = 0; j < VW; ++j) {
                 const float u = V::get(q[R][r], j);
@@ -275,31 +283,53 @@ fd_step_kernel(const FdStepArgs a) {
                 const float p = V::get(pc[r], j);
                 const float d = damp_prev ? fminf(dxs, dyz[r][j]) : 1.0f;
                 const float c = V::get(cc[r], j);
-                float nv = u + (fmaf(-d, p, u) + c * lap);
-                if (!finitef(nv) && rvalid[r]) {
-                    // reference guard order (wave_equation.cpp:345-397): a
-                    // non-finite centre or previous value gives 0, a non-finite
-                    // neighbour keeps the centre value (both unclamped), a
-                    // non-finite result keeps the centre value (clamped).
-                    guard_hit = true;
-                    const int64_t ci = pb + (roff[r] - z) + (z + j);
-                    bool ok_n = true;
+                const float nv = u + (fmaf(-d, p, u) + c * lap);
+                bad |= !finitef(nv) && rvalid[r];
+                V::set(res[r], j, fminf(fmaxf(nv, -amp), amp));
+            }
+        }
+        if (bad) {
+            // rare path, reference guard order (wave_equation.cpp:345-397): a
+            // non-finite centre or previous value gives 0, a non-finite
+            // neighbour keeps the centre value (both unclamped), a non-finite
+            // result keeps the centre value (clamped).
+            guard_hit = true;
 #pragma unroll
+            for (int r = 0; r < TYR; ++r) {
+#pragma unroll
+                for (int j = 0; j < VW; ++j) {
+                    const float u = V::get(q[R][r], j);
+                    const float p = V::get(pc[r], j);
+                    const float c = V::get(cc[r], j);
+                    const float d = damp_prev ? fminf(dxs, dyz[r][j]) : 1.0f;
+                    if (!rvalid[r]) continue;
+                    const uint32_t ci = pb + roff[r] + j;
+                    bool ok_n = true;
+                    float lap = 0.f;
                     for (int k = 1; k <= R; ++k) {
-                        ok_n &= finitef(U[ci + k * plane]) && finitef(U[ci - k * plane]);
-                        ok_n &= finitef(U[ci + k * nzh]) && finitef(U[ci - k * nzh]);
-                        ok_n &= finitef(U[ci + k]) && finitef(U[ci - k]);
+                        const float xp = U[ci + k * plane], xm = U[ci - k * plane];
+                        const float yp = U[ci + k * nzh], ym = U[ci - k * nzh];
+                        const float zp = U[ci + k], zm = U[ci - k];
+                        ok_n &= finitef(xp) && finitef(xm) && finitef(yp) && finitef(ym) && finitef(zp) &&
+                                finitef(zm);
+                        lap = fmaf(a.cx[k], (xp - u) + (xm - u), lap);
+                        lap = fmaf(a.cy[k], (yp - u) + (ym - u), lap);
+                        lap = fmaf(a.cz[k], (zp - u) + (zm - u), lap);
                     }
+                    float nv;
                     if (!(finitef(u) && finitef(p))) nv = 0.f;
                     else if (!ok_n) nv = u;
-                    else nv = fminf(fmaxf(u, -amp), amp);
-                } else {
-                    nv = fminf(fmaxf(nv, -amp), amp);
+                    else {
+                        nv = u + (fmaf(-d, p, u) + c * lap);
+                        nv = fminf(fmaxf(finitef(nv) ? nv : u, -amp), amp);
+                    }
+                    V::set(res[r], j, nv);
                 }
-                V::set(out, j, nv);
             }
-            if (rvalid[r]) *reinterpret_cast<vec*>(P + pb + roff[r]) = out;
         }
+#pragma unroll
+        for (int r = 0; r < TYR; ++r)
+            if (rvalid[r]) *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
     }
 
     if (guard_hit && a.step_status) atomicOr(a.step_status, STP_STEP_NAN_DETECTED_AND_REPAIRED_DEV);
@@ -321,7 +351,7 @@ fd_step_kernel(const FdStepArgs a) {
         __syncthreads();   // receivers read the pre-injection value
         if (tid == 0) {
             const int64_t s = a.src_idx[shot];
-            const int sx = (int)(s / plane), sy = (int)((s / nzh) % nyh), sz = (int)(s % nzh);
+            const int sx = (int)(s / (int64_t)plane), sy = (int)((s / nzh) % nyh), sz = (int)(s % nzh);
             if (sx >= x0 && sx < x1 && sy >= yb && sy < yb + TY && sz >= zb && sz < zb + TZ) {
                 const float w = a.w_next[(int64_t)shot * a.w_stride];
                 if (finitef(w)) P[s] += w;
@@ -380,18 +410,43 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, hipStream_t st) 
     return hipGetLastError();
 }
 
-void fd_tiling(int R, int VW, int* tz, int* ty, int* lx) {
+// Tile shapes (TZ, TY, TYR, LX) compiled for the float4 path; variant 0 is
+// the default, the others are kept for tuning (STP_FD_TILE).
+#define STP_FD_VARIANTS(X)       \
+    X(0, 64, 16, 2, 64)          \
+    X(1, 64, 16, 1, 64)          \
+    X(2, 64, 32, 2, 64)          \
+    X(3, 128, 16, 2, 64)         \
+    X(4, 64, 16, 2, 128)         \
+    X(5, 64, 32, 4, 64)          \
+    X(6, 128, 8, 1, 64)          \
+    X(7, 64, 8, 1, 128)
+
+int fd_num_variants() { return 8; }
+
+void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
     (void)R;
-    if (VW == 4) { *tz = 64; *ty = 16; *lx = 64; }
-    else { *tz = 64; *ty = 8; *lx = 64; }
+    if (VW == 1) { *tz = 64; *ty = 8; *lx = 64; return; }
+#define STP_TILE_CASE(id, TZ, TY, TYR, LX) \
+    if (variant == id) { *tz = TZ; *ty = TY; *lx = LX; return; }
+    STP_FD_VARIANTS(STP_TILE_CASE)
+#undef STP_TILE_CASE
+    *tz = 64; *ty = 16; *lx = 64;
 }
 
-hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, hipStream_t st) {
+hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipStream_t st) {
     const int nblocks = a.ntz * a.nty * a.nxc * a.nshots;
-    if (R == 4 && VW == 4) return launch_cfg<4, 4, 64, 16, 2, 64>(a, nblocks, st);
-    if (R == 4 && VW == 1) return launch_cfg<4, 1, 64, 8, 2, 64>(a, nblocks, st);
-    if (R == 1 && VW == 4) return launch_cfg<1, 4, 64, 16, 2, 64>(a, nblocks, st);
-    if (R == 1 && VW == 1) return launch_cfg<1, 1, 64, 8, 2, 64>(a, nblocks, st);
+    if (VW == 1) {
+        if (R == 4) return launch_cfg<4, 1, 64, 8, 2, 64>(a, nblocks, st);
+        if (R == 1) return launch_cfg<1, 1, 64, 8, 2, 64>(a, nblocks, st);
+        return hipErrorInvalidValue;
+    }
+    if (R == 1) return launch_cfg<1, 4, 64, 16, 2, 64>(a, nblocks, st);
+    if (R != 4) return hipErrorInvalidValue;
+#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX) \
+    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX>(a, nblocks, st);
+    STP_FD_VARIANTS(STP_LAUNCH_CASE)
+#undef STP_LAUNCH_CASE
     return hipErrorInvalidValue;
 }
 

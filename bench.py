@@ -42,7 +42,7 @@ def pmc_traffic(n: int) -> tuple[float | None, str | None]:
     return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
-def cpu_baseline(threads: int, steps: int = 100):
+def cpu_baseline(threads: int, steps: int = 500):
     """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
     on the 128^3 two-layer case for `steps` of its 500 steps."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -69,6 +69,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for PMC byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                     or min(16, os.cpu_count() or 1))
     args = ap.parse_args()
@@ -82,6 +84,14 @@ def main() -> int:
     from seismic_engine import FDPropagator, synthetic
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 << 28, "bytes_written": 4 << 28, "copies": 3}))
+        return 0
     n, nt, hw = args.n, args.nt, 4
     log(f"building {n}^3 velocity (seed 2026, sigma 20) on the GPU")
     vel = synthetic.smooth_random_velocity(n, n, n, seed=2026, sigma=20.0, device=dev, as_tensor=True)
@@ -155,7 +165,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         del fd
         torch.cuda.empty_cache()
-        steps = 100
+        steps = 500
         log(f"cpu baseline: FP64 oracle 128^3, {steps} steps, {args.cpu_threads} threads")
         gcs, sec, cur_o, tr_o, v_h, w_np, rcv1 = cpu_baseline(args.cpu_threads, steps)
         result["cpu_baseline"] = {

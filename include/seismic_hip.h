@@ -177,6 +177,30 @@ int stp_cg(const stp_csr* A, const double* b_dev, int32_t max_iter, double tol, 
 int stp_cgls(const stp_csr* A, const double* b_dev, double lambda, double tol, int32_t max_iter, int32_t precond,
              void* precond_handle, double* x_dev, double* history_dev, stp_solve_info* info, void* stream);
 
+/* Canonical CSR assembly, CSRMatrix::build_from_triplets semantics
+ * (csr_matrix.cpp:25-72) without its padding pass: triplets whose row or
+ * column is out of range are dropped, rows are ordered by column, duplicate
+ * (row, col) entries are summed in input order.  Device inputs r, c (int64),
+ * v (FP64) of length n; device outputs row_ptr (rows + 1), col_out and
+ * val_out (capacity n); *nnz_host = merged entry count.  Synchronises. */
+int stp_csr_assemble(int64_t rows, int64_t cols, int64_t n, const int64_t* r_dev, const int64_t* c_dev,
+                     const double* v_dev, int64_t* row_ptr_dev, int64_t* col_out_dev, double* val_out_dev,
+                     int64_t* nnz_host, void* stream);
+/* Triplets of the 7-point FD matrix (build_fd_stencil_7point,
+ * csr_matrix.cpp:74-116): 7 slots per cell, absent neighbours get row -1. */
+int stp_csr64_fd7_triplets(int64_t nx, int64_t ny, int64_t nz, const double* vel_dev, double dx, double dy,
+                           double dz, double dt, int64_t* r_dev, int64_t* c_dev, double* v_dev, void* stream);
+/* CSRMatrix::scale / add_diagonal (first entry with col == row) on FP64 CSR */
+int stp_csr64_scale(int64_t nnz, double* val_dev, double alpha, void* stream);
+int stp_csr64_add_diagonal(int64_t rows, const int64_t* row_ptr_dev, const int64_t* col_dev, double* val_dev,
+                           double alpha, void* stream);
+/* FP64 dot product (SparseOps::dot_product, sparse_ops.cpp:172-187; tree order) */
+int stp_dot_f64(int64_t n, const double* a_dev, const double* b_dev, double* out_host, void* stream);
+/* FP64/int64 CSR -> FP32/int32 arrays for stp_csr_create */
+int stp_csr64_to_f32(int64_t rows, int64_t nnz, const int64_t* row_ptr_dev, const int64_t* col_dev,
+                     const double* val_dev, int32_t* row_ptr32_dev, int32_t* col32_dev, float* val32_dev,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif
