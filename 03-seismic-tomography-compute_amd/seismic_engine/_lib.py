@@ -28,6 +28,14 @@ class FdConfig(C.Structure):
     ]
 
 
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("rel_residual", C.c_double),
+                ("true_rel_residual", C.c_double), ("replacements", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _vp = C.c_void_p
 _i32 = C.c_int32
 _i64 = C.c_int64
@@ -55,6 +63,21 @@ SIGNATURES: dict[str, list] = {
     "stp_count_nonfinite_f64": [_vp, _i64, C.POINTER(_i64), _vp],
     "stp_clamp_field_f64": [_vp, _i64, _f64, _f64, _vp],
     "stp_repair_field_nan_f64": [_vp, _i64, _i64, _i64, _i64, _i32, _f64, _vp],
+    "stp_csr_create": [_i64, _i64, _i64, _vp, _vp, _vp, C.POINTER(_vp), _vp],
+    "stp_csr_from_rays": [_i64, _vp, _f64, _vp, C.POINTER(_vp), _vp],
+    "stp_csr_destroy": [_vp],
+    "stp_csr_shape": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
+    "stp_csr_export": [_vp, _i32, _vp, _vp, _vp, _vp],
+    "stp_csr_spmv": [_vp, _i32, _vp, _vp, _vp],
+    "stp_csr_spmv_f64": [_vp, _i32, _vp, _vp, _vp],
+    "stp_cg": [_vp, _vp, _i32, _f64, _i32, _vp, _vp, C.POINTER(SolveInfo), _vp],
+    "stp_cgls": [_vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _vp, C.POINTER(SolveInfo), _vp],
+    "stp_csr_assemble": [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64), _vp],
+    "stp_csr64_fd7_triplets": [_i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp],
+    "stp_csr64_scale": [_i64, _vp, _f64, _vp],
+    "stp_csr64_add_diagonal": [_i64, _vp, _vp, _vp, _f64, _vp],
+    "stp_dot_f64": [_i64, _vp, _vp, C.POINTER(_f64), _vp],
+    "stp_csr64_to_f32": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

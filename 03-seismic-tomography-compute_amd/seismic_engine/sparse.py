@@ -13,7 +13,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import call, lib, ptr, stream_handle
+from ._lib import SolveInfo, call, lib, ptr, stream_handle
 
 
 def _torch():
@@ -35,14 +35,6 @@ def _as_dev(a, dtype):
 def _out(t, like):
     torch = _torch()
     return t if torch.is_tensor(like) else t.cpu().numpy()
-
-
-class SolveInfo(C.Structure):
-    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("rel_residual", C.c_double),
-                ("true_rel_residual", C.c_double), ("replacements", C.c_int32)]
-
-    def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class DeviceCSR:
