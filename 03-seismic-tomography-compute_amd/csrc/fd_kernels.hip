@@ -122,16 +122,20 @@ fd_step_kernel(const FdStepArgs a) {
     const uint32_t plane = (uint32_t)nyh * (uint32_t)nzh;
     const uint32_t safe = (uint32_t)yb * nzh + (uint32_t)zb;
 
-    // own cells
+    // own cells: rvalid = interior (stored); positions inside the array but
+    // outside the interior are halo cells whose (zero) values the valid
+    // neighbours read, so they load from their true address; only positions
+    // beyond the array use the clamped address (no valid cell reads those).
     const int z = zb + tz * VW;
     const bool zvalid = z < nzh - hw;
+    const bool zinb = z + VW <= nzh;
     bool rvalid[TYR];
-    uint32_t roff[TYR];   // offset of own vector within a plane
+    uint32_t roff[TYR];   // load offset of own vector within a plane
 #pragma unroll
     for (int r = 0; r < TYR; ++r) {
         const int y = yb + ty * TYR + r;
         rvalid[r] = zvalid && (y < nyh - hw);
-        roff[r] = rvalid[r] ? (uint32_t)y * nzh + (uint32_t)z : safe;
+        roff[r] = (zinb && y < nyh) ? (uint32_t)y * nzh + (uint32_t)z : safe;
     }
 
     // halo items: LDS slot (-1 = none) and in-plane offset
