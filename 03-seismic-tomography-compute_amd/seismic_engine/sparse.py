@@ -57,9 +57,10 @@ class DeviceCSR:
         return cls.from_device(rows, cols, rp, ci, v)
 
     @classmethod
-    def from_device(cls, rows, cols, rp32, ci32, v32):
+    def from_device(cls, rows, cols, rp32, ci32, v32, nnz=None):
         h = C.c_void_p()
-        call("stp_csr_create", C.c_int64(rows), C.c_int64(cols), C.c_int64(ci32.numel()), ptr(rp32), ptr(ci32),
+        nnz = ci32.numel() if nnz is None else nnz
+        call("stp_csr_create", C.c_int64(rows), C.c_int64(cols), C.c_int64(nnz), ptr(rp32), ptr(ci32),
              ptr(v32), C.byref(h), stream_handle())
         return cls(h)
 
@@ -186,8 +187,7 @@ class CSRMatrix:
             v32 = torch.empty(max(nnz, 1), dtype=torch.float32, device=_dev())
             call("stp_csr64_to_f32", C.c_int64(self._rows), C.c_int64(nnz), ptr(self._rp), ptr(self._ci),
                  ptr(self._val), ptr(rp32), ptr(ci32), ptr(v32), stream_handle())
-            self._dev = DeviceCSR.from_device(self._rows, self._cols, rp32, ci32[:nnz] if nnz else ci32,
-                                              v32[:nnz] if nnz else v32)
+            self._dev = DeviceCSR.from_device(self._rows, self._cols, rp32, ci32, v32, nnz)
         return self._dev
 
     # ---- reference methods
