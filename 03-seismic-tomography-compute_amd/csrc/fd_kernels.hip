@@ -35,14 +35,21 @@
 namespace stp {
 
 template <int VW> struct Vec;
+typedef float f4n __attribute__((ext_vector_type(4)));
 template <> struct Vec<1> {
     using T = float;
+    using N = float;
+    __device__ static inline T from_native(N v) { return v; }
+    __device__ static inline N to_native(T v) { return v; }
     __device__ static inline float get(const T& v, int) { return v; }
     __device__ static inline void set(T& v, int, float x) { v = x; }
     __device__ static inline T zero() { return 0.f; }
 };
 template <> struct Vec<4> {
     using T = float4;
+    using N = f4n;
+    __device__ static inline T from_native(N v) { return make_float4(v.x, v.y, v.z, v.w); }
+    __device__ static inline N to_native(T v) { return f4n{v.x, v.y, v.z, v.w}; }
     __device__ static inline float get(const T& v, int j) {
         return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
     }
@@ -79,7 +86,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-template <int R, int VW, int TZ, int TY, int TYR, int LX>
+template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL>
 __global__ void __launch_bounds__((TZ / VW) * (TY / TYR))
 fd_step_kernel(const FdStepArgs a) {
     using V = Vec<VW>;
@@ -183,6 +190,17 @@ fd_step_kernel(const FdStepArgs a) {
     auto ldv = [](const float* base, uint32_t off) -> vec {
         return *reinterpret_cast<const vec*>(base + off);
     };
+    // streamed-once operands (P, C): optionally non-temporal so they do not
+    // evict the U planes whose halos neighbouring tiles re-read from L2
+    auto ldn = [](const float* base, uint32_t off) -> vec {
+        if constexpr (NTL) {
+            using nv_t = typename Vec<VW>::N;
+            const nv_t v = __builtin_nontemporal_load(reinterpret_cast<const nv_t*>(base + off));
+            return V::from_native(v);
+        } else {
+            return *reinterpret_cast<const vec*>(base + off);
+        }
+    };
 
     // ---- warm-up: queue planes x0-R .. x0+R-1 into q[1..2R]
     vec q[2 * R + 1][TYR];
@@ -200,8 +218,8 @@ fd_step_kernel(const FdStepArgs a) {
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
             qn[r] = ldv(U, pf + roff[r]);
-            pn[r] = ldv(P, pb + roff[r]);
-            cn[r] = ldv(C, pb + roff[r]);
+            pn[r] = ldn(P, pb + roff[r]);
+            cn[r] = ldn(C, pb + roff[r]);
         }
 #pragma unroll
         for (int m = 0; m < NH; ++m) hn[m] = ldv(U, pb + hoff[m]);
@@ -238,8 +256,8 @@ fd_step_kernel(const FdStepArgs a) {
 #pragma unroll
             for (int r = 0; r < TYR; ++r) {
                 qn[r] = ldv(U, pf + roff[r]);
-                pn[r] = ldv(P, pb + roff[r]);
-                cn[r] = ldv(C, pb + roff[r]);
+                pn[r] = ldn(P, pb + roff[r]);
+                cn[r] = ldn(C, pb + roff[r]);
             }
 #pragma unroll
             for (int m = 0; m < NH; ++m) hn[m] = ldv(U, pb + hoff[m]);
@@ -333,7 +351,12 @@ fd_step_kernel(const FdStepArgs a) {
         }
 #pragma unroll
         for (int r = 0; r < TYR; ++r)
-            if (rvalid[r]) *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
+            if (rvalid[r]) {
+                if constexpr (NTL)
+                    __builtin_nontemporal_store(V::to_native(res[r]),
+                                                reinterpret_cast<typename Vec<VW>::N*>(P + pb + roff[r]));
+                else *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
+            }
     }
 
     if (guard_hit && a.step_status) atomicOr(a.step_status, STP_STEP_NAN_DETECTED_AND_REPAIRED_DEV);
@@ -407,35 +430,39 @@ __global__ void fd_sponge_kernel(float* d, int nxh, int nyh, int nzh, int hw, in
 }
 
 // ------------------------------------------------------------------ launchers
-template <int R, int VW, int TZ, int TY, int TYR, int LX>
+template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL = false>
 static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, hipStream_t st) {
     constexpr int NT = (TZ / VW) * (TY / TYR);
-    hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX>), dim3(nblocks), dim3(NT), 0, st, a0);
+    hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL>), dim3(nblocks), dim3(NT), 0, st, a0);
     return hipGetLastError();
 }
 
 // Tile shapes (TZ, TY, TYR, LX) compiled for the float4 path; variant 0 is
 // the default, the others are kept for tuning (STP_FD_TILE).
-#define STP_FD_VARIANTS(X)       \
-    X(0, 64, 16, 2, 64)          \
-    X(1, 64, 16, 1, 64)          \
-    X(2, 64, 32, 2, 64)          \
-    X(3, 128, 16, 2, 64)         \
-    X(4, 64, 16, 2, 128)         \
-    X(5, 64, 32, 4, 64)          \
-    X(6, 128, 8, 1, 64)          \
-    X(7, 64, 8, 1, 128)
+#define STP_FD_VARIANTS(X)            \
+    X(0, 128, 8, 1, 64, false)        \
+    X(1, 64, 16, 1, 64, false)        \
+    X(2, 128, 8, 1, 128, false)       \
+    X(3, 128, 16, 2, 64, false)       \
+    X(4, 64, 16, 2, 128, false)       \
+    X(5, 128, 8, 1, 64, true)         \
+    X(6, 128, 16, 1, 64, false)       \
+    X(7, 128, 16, 1, 128, true)       \
+    X(8, 128, 8, 1, 128, true)        \
+    X(9, 256, 8, 1, 64, false)        \
+    X(10, 128, 12, 1, 96, false)      \
+    X(11, 64, 16, 2, 64, false)
 
-int fd_num_variants() { return 8; }
+int fd_num_variants() { return 12; }
 
 void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
     (void)R;
     if (VW == 1) { *tz = 64; *ty = 8; *lx = 64; return; }
-#define STP_TILE_CASE(id, TZ, TY, TYR, LX) \
+#define STP_TILE_CASE(id, TZ, TY, TYR, LX, NTL) \
     if (variant == id) { *tz = TZ; *ty = TY; *lx = LX; return; }
     STP_FD_VARIANTS(STP_TILE_CASE)
 #undef STP_TILE_CASE
-    *tz = 64; *ty = 16; *lx = 64;
+    *tz = 128; *ty = 8; *lx = 64;
 }
 
 hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipStream_t st) {
@@ -447,8 +474,8 @@ hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipSt
     }
     if (R == 1) return launch_cfg<1, 4, 64, 16, 2, 64>(a, nblocks, st);
     if (R != 4) return hipErrorInvalidValue;
-#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX) \
-    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX>(a, nblocks, st);
+#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL) \
+    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL>(a, nblocks, st);
     STP_FD_VARIANTS(STP_LAUNCH_CASE)
 #undef STP_LAUNCH_CASE
     return hipErrorInvalidValue;

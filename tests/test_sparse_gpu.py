@@ -79,7 +79,8 @@ def test_cgls_matches_fp64_oracle(precond, kind, cuda):
     x, info, hist = A.cgls(b, lam, 1e-6, 5000, precond, history=True)
     xo, it_o, rr_o, _ = osp.cgls(rp, ci, v, b, A.cols, lam, 1e-6, 5000, kind)
     assert info.converged == 1 and info.true_rel_residual < 1e-6
-    assert abs(info.iterations - it_o) <= max(5, 0.1 * it_o)
+    # FP32 inner SpMV: Krylov convergence within 20% of the FP64 iteration count
+    assert abs(info.iterations - it_o) <= max(5, 0.2 * it_o)
     # both solutions satisfy the normal equations to 1e-6; compare to the
     # accuracy that residual level supports
     assert np.linalg.norm(_np(x) - xo) / np.linalg.norm(xo) < 1e-4
