@@ -455,9 +455,17 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, hipStream_t st) 
 
 int fd_num_variants() { return 12; }
 
+// Fixed tiles of the other paths (must match fd_launch_step below).
+#define STP_FD_FIXED(X)                 \
+    X(4, 1, 64, 8, 2, 64)               \
+    X(1, 1, 64, 8, 2, 64)               \
+    X(1, 4, 64, 16, 2, 64)
+
 void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
-    (void)R;
-    if (VW == 1) { *tz = 64; *ty = 8; *lx = 64; return; }
+#define STP_FIXED_TILE(RR, VV, TZ, TY, TYR, LX) \
+    if (R == RR && VW == VV) { *tz = TZ; *ty = TY; *lx = LX; return; }
+    STP_FD_FIXED(STP_FIXED_TILE)
+#undef STP_FIXED_TILE
 #define STP_TILE_CASE(id, TZ, TY, TYR, LX, NTL) \
     if (variant == id) { *tz = TZ; *ty = TY; *lx = LX; return; }
     STP_FD_VARIANTS(STP_TILE_CASE)
@@ -467,13 +475,11 @@ void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
 
 hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipStream_t st) {
     const int nblocks = a.ntz * a.nty * a.nxc * a.nshots;
-    if (VW == 1) {
-        if (R == 4) return launch_cfg<4, 1, 64, 8, 2, 64>(a, nblocks, st);
-        if (R == 1) return launch_cfg<1, 1, 64, 8, 2, 64>(a, nblocks, st);
-        return hipErrorInvalidValue;
-    }
-    if (R == 1) return launch_cfg<1, 4, 64, 16, 2, 64>(a, nblocks, st);
-    if (R != 4) return hipErrorInvalidValue;
+#define STP_FIXED_LAUNCH(RR, VV, TZ, TY, TYR, LX) \
+    if (R == RR && VW == VV) return launch_cfg<RR, VV, TZ, TY, TYR, LX>(a, nblocks, st);
+    STP_FD_FIXED(STP_FIXED_LAUNCH)
+#undef STP_FIXED_LAUNCH
+    if (R != 4 || VW != 4) return hipErrorInvalidValue;
 #define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL) \
     if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL>(a, nblocks, st);
     STP_FD_VARIANTS(STP_LAUNCH_CASE)
