@@ -81,9 +81,13 @@ def test_cgls_matches_fp64_oracle(precond, kind, cuda):
     assert info.converged == 1 and info.true_rel_residual < 1e-6
     # FP32 inner SpMV: Krylov convergence within 20% of the FP64 iteration count
     assert abs(info.iterations - it_o) <= max(5, 0.2 * it_o)
-    # both solutions satisfy the normal equations to 1e-6; compare to the
-    # accuracy that residual level supports
-    assert np.linalg.norm(_np(x) - xo) / np.linalg.norm(xo) < 1e-4
+    # error against the tightly converged FP64 solution: no worse than the
+    # FP64 oracle stopped at the same tolerance (a relative residual of 1e-6
+    # only bounds the error through the condition number)
+    xs, _, _, _ = osp.cgls(rp, ci, v, b, A.cols, lam, 1e-13, 20000, 0)
+    e_gpu = np.linalg.norm(_np(x) - xs)
+    e_orc = np.linalg.norm(xo - xs)
+    assert e_gpu <= 3.0 * e_orc + 1e-9 * np.linalg.norm(xs)
     h = _np(hist)
     assert h[-1] < 1e-6 and np.all(h[:-1] >= 1e-6)
 
