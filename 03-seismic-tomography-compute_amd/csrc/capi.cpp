@@ -15,6 +15,7 @@
 #include "fd_kernels.h"
 #include "health_kernels.h"
 #include "capi_common.h"
+#include "fd_plan.h"
 #include "seismic_hip.h"
 
 namespace stp {
@@ -40,22 +41,28 @@ int fd_weights(int order, double* a) {
 
 }  // namespace
 
-struct stp_fd_plan {
-    stp_fd_config cfg;
-    int R = 0, VW = 1, variant = 0, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
-    int nxh = 0, nyh = 0, nzh = 0;
-    int64_t ncell = 0;
-    float cx[5] = {0}, cy[5] = {0}, cz[5] = {0};
-    // receivers
-    int nrcv = 0, nrcv_pts = 0;
-    int* d_blk_ptr = nullptr;
-    int64_t* d_rcv_cell = nullptr;
-    int* d_rcv_id = nullptr;
-    // sources
-    int64_t* d_src = nullptr;
-    // scratch
-    unsigned int* d_word = nullptr;
-};
+namespace stp {
+// common launch arguments of a plan (pointers and per-step fields left zero)
+void fd_fill_args(const stp_fd_plan* p, FdStepArgs* a) {
+    const stp_fd_config& c = p->cfg;
+    std::memset(a, 0, sizeof *a);
+    a->ncell = p->ncell;
+    a->nxh = p->nxh; a->nyh = p->nyh; a->nzh = p->nzh; a->hw = c.halo_width;
+    std::memcpy(a->cx, p->cx, sizeof a->cx);
+    std::memcpy(a->cy, p->cy, sizeof a->cy);
+    std::memcpy(a->cz, p->cz, sizeof a->cz);
+    a->sw = c.sponge_width;
+    a->sigma_over_sw2 = c.sponge_width > 0 ? (float)(c.sponge_damping / ((double)c.sponge_width * c.sponge_width)) : 0.f;
+    a->amp = (float)c.amplitude_limit;
+    a->ntz = p->ntz; a->nty = p->nty; a->nxc = p->nxc; a->nshots = c.nshots;
+    a->nrcv = p->nrcv;
+    a->blk_rcv_ptr = p->d_blk_ptr;
+    a->rcv_cell = p->d_rcv_cell;
+    a->rcv_id = p->d_rcv_id;
+    a->nx = (int)c.nx; a->ny = (int)c.ny; a->nz = (int)c.nz;
+    a->nint = c.nx * c.ny * c.nz;
+}
+}  // namespace stp
 
 extern "C" {
 
@@ -269,22 +276,9 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
     if (nsteps == 0) return STP_OK;
     const stp_fd_config& c = p->cfg;
     stp::FdStepArgs a;
-    std::memset(&a, 0, sizeof a);
+    stp::fd_fill_args(p, &a);
     a.coef = coef;
-    a.ncell = p->ncell;
-    a.nxh = p->nxh; a.nyh = p->nyh; a.nzh = p->nzh; a.hw = c.halo_width;
-    std::memcpy(a.cx, p->cx, sizeof a.cx);
-    std::memcpy(a.cy, p->cy, sizeof a.cy);
-    std::memcpy(a.cz, p->cz, sizeof a.cz);
-    a.sw = c.sponge_width;
-    a.sigma_over_sw2 = c.sponge_width > 0 ? (float)(c.sponge_damping / ((double)c.sponge_width * c.sponge_width)) : 0.f;
-    a.amp = (float)c.amplitude_limit;
-    a.ntz = p->ntz; a.nty = p->nty; a.nxc = p->nxc; a.nshots = c.nshots;
-    a.nrcv = p->nrcv;
     a.nrcv_pts = traces ? p->nrcv_pts : 0;
-    a.blk_rcv_ptr = p->d_blk_ptr;
-    a.rcv_cell = p->d_rcv_cell;
-    a.rcv_id = p->d_rcv_id;
     a.traces = traces;
     a.trace_stride = trace_stride;
     const bool inject = wavelet && nw > 0 && p->d_src;
@@ -302,7 +296,7 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
         a.t_col = s;
         a.w_next = (inject && s + 1 < nsteps && t + 1 < nw) ? wavelet + (t + 1) : nullptr;
         a.step_status = status ? status + s : nullptr;
-        STP_HIP(stp::fd_launch_step(a, p->R, p->VW, p->variant, S(st)));
+        STP_HIP(stp::fd_launch_step(a, p->R, p->VW, p->variant, stp::FD_MODE_PLAIN, S(st)));
         std::swap(u, v);
     }
     return STP_OK;

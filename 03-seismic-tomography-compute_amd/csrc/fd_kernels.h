@@ -34,11 +34,30 @@ struct FdStepArgs {
     const float* w_next;      // &wavelet[t+1] of shot 0; shot s at s*w_stride
     int64_t w_stride;
     int* step_status;         // OR-ed with STP_STEP_NAN_DETECTED_AND_REPAIRED
+    // mode 1 (forward recompute): per-shot with-halo snapshot of the step's
+    // input field (snap_lap = 0) or of its Laplacian L(u) (snap_lap = 1)
+    float* snap_out;
+    int snap_lap;
+    // mode 2 (backward pass): acc_out += N * acc_src, acc_ill += acc_src^2
+    // (interior-indexed, per shot), then the receivers inject
+    // inj_traces[(shot*nrcv + r)*inj_stride + inj_col] (times C at the
+    // receiver when inj_scale_c) into N
+    const float* acc_src;
+    float* acc_out;
+    float* acc_ill;
+    int64_t nint;             // interior cells per shot
+    int nx, ny, nz;           // interior extents
+    const float* inj_traces;
+    int64_t inj_stride;
+    int inj_col;
+    int inj_scale_c;
 };
+
+enum { FD_MODE_PLAIN = 0, FD_MODE_SNAPSHOT = 1, FD_MODE_ACCUMULATE = 2 };
 
 int fd_num_variants();
 void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx);
-hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipStream_t st);
+hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, int mode, hipStream_t st);
 hipError_t fd_launch_inject(float* u, int64_t ncell, const int64_t* src_idx, const float* w, int64_t w_stride,
                             int nshots, hipStream_t st);
 hipError_t fd_launch_coef(const float* vel, float* coef, int64_t n, float dt2, float damp, hipStream_t st);

@@ -86,7 +86,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL>
+template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL, int MODE>
 __global__ void __launch_bounds__((TZ / VW) * (TY / TYR))
 fd_step_kernel(const FdStepArgs a) {
     using V = Vec<VW>;
@@ -267,6 +267,7 @@ fd_step_kernel(const FdStepArgs a) {
         const float dxs = sponge_of(sponge_dist(x, nxh, hw, sw), sw, sg);
         const uint32_t pb = (uint32_t)x * plane;
         vec res[TYR];
+        vec lapv[TYR];
         bool bad = false;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
@@ -306,6 +307,7 @@ fd_step_kernel(const FdStepArgs a) {
                 const float d = damp_prev ? fminf(dxs, dyz[r][j]) : 1.0f;
                 const float c = V::get(cc[r], j);
                 const float nv = u + (fmaf(-d, p, u) + c * lap);
+                if constexpr (MODE == FD_MODE_SNAPSHOT) V::set(lapv[r], j, lap);
                 bad |= !finitef(nv) && rvalid[r];
                 V::set(res[r], j, fminf(fmaxf(nv, -amp), amp));
             }
@@ -356,6 +358,31 @@ fd_step_kernel(const FdStepArgs a) {
                     __builtin_nontemporal_store(V::to_native(res[r]),
                                                 reinterpret_cast<typename Vec<VW>::N*>(P + pb + roff[r]));
                 else *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
+                if constexpr (MODE == FD_MODE_SNAPSHOT) {
+                    float* snap = a.snap_out + (int64_t)shot * a.ncell;
+                    *reinterpret_cast<vec*>(snap + pb + roff[r]) = a.snap_lap ? lapv[r] : q[R][r];
+                }
+                if constexpr (MODE == FD_MODE_ACCUMULATE) {
+                    const float* S = a.acc_src + (int64_t)shot * a.ncell;
+                    const vec sv = ldv(S, pb + roff[r]);
+                    const int y = yb + ty * TYR + r;
+                    const int64_t ii = (int64_t)shot * a.nint +
+                                       ((int64_t)(x - hw) * a.ny + (y - hw)) * a.nz + (z - hw);
+                    vec* ao = reinterpret_cast<vec*>(a.acc_out + ii);
+                    vec acc = *ao;
+#pragma unroll
+                    for (int j = 0; j < VW; ++j)
+                        V::set(acc, j, fmaf(V::get(res[r], j), V::get(sv, j), V::get(acc, j)));
+                    *ao = acc;
+                    if (a.acc_ill) {
+                        vec* io = reinterpret_cast<vec*>(a.acc_ill + ii);
+                        vec ill = *io;
+#pragma unroll
+                        for (int j = 0; j < VW; ++j)
+                            V::set(ill, j, fmaf(V::get(sv, j), V::get(sv, j), V::get(ill, j)));
+                        *io = ill;
+                    }
+                }
             }
     }
 
@@ -366,6 +393,27 @@ fd_step_kernel(const FdStepArgs a) {
     if (a.nrcv_pts == 0 && a.src_idx == nullptr) return;
     __syncthreads();
     const int tile = lb % (a.ntz * a.nty * a.nxc);
+    if constexpr (MODE == FD_MODE_ACCUMULATE) {
+        if (a.inj_traces) {
+            // receiver injection into the new field (atomic: two receivers may
+            // share a cell); the accumulated correlation gets the matching
+            // correction so it sees the post-injection value
+            const float* S = a.acc_src + (int64_t)shot * a.ncell;
+            const int pbeg = a.blk_rcv_ptr[tile], pend = a.blk_rcv_ptr[tile + 1];
+            for (int p = pbeg + tid; p < pend; p += NT) {
+                const int64_t ci = a.rcv_cell[p];
+                const int rid = a.rcv_id[p];
+                float amt = a.inj_traces[((int64_t)shot * a.nrcv + rid) * a.inj_stride + a.inj_col];
+                if (a.inj_scale_c) amt *= C[ci];
+                if (!finitef(amt)) continue;
+                atomicAdd(&P[ci], amt);
+                const int cx = (int)(ci / (int64_t)plane), cy = (int)((ci / nzh) % nyh), cz = (int)(ci % nzh);
+                const int64_t ii = (int64_t)shot * a.nint + ((int64_t)(cx - hw) * a.ny + (cy - hw)) * a.nz + (cz - hw);
+                atomicAdd(&a.acc_out[ii], amt * S[ci]);
+            }
+        }
+        return;
+    }
     if (a.traces) {
         const int pbeg = a.blk_rcv_ptr[tile], pend = a.blk_rcv_ptr[tile + 1];
         for (int p = pbeg + tid; p < pend; p += NT) {
@@ -431,14 +479,24 @@ __global__ void fd_sponge_kernel(float* d, int nxh, int nyh, int nzh, int hw, in
 
 // ------------------------------------------------------------------ launchers
 template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL = false>
-static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, hipStream_t st) {
+static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStream_t st) {
     constexpr int NT = (TZ / VW) * (TY / TYR);
-    hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL>), dim3(nblocks), dim3(NT), 0, st, a0);
+    if (mode == FD_MODE_SNAPSHOT)
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_SNAPSHOT>), dim3(nblocks), dim3(NT), 0,
+                           st, a0);
+    else if (mode == FD_MODE_ACCUMULATE)
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_ACCUMULATE>), dim3(nblocks), dim3(NT),
+                           0, st, a0);
+    else
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_PLAIN>), dim3(nblocks), dim3(NT), 0, st,
+                           a0);
     return hipGetLastError();
 }
 
-// Tile shapes (TZ, TY, TYR, LX) compiled for the float4 path; variant 0 is
-// the default, the others are kept for tuning (STP_FD_TILE).
+// Tile shapes (TZ, TY, TYR, LX, non-temporal P/C/N) compiled for the float4
+// path.  Variant 6 (128 x 16 cells, one row per thread, 512 threads) is the
+// default (fastest measured at 512^3); the others are kept for tuning
+// through STP_FD_TILE=<index>.
 #define STP_FD_VARIANTS(X)            \
     X(0, 128, 8, 1, 64, false)        \
     X(1, 64, 16, 1, 64, false)        \
@@ -473,15 +531,15 @@ void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
     *tz = 128; *ty = 8; *lx = 64;
 }
 
-hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, hipStream_t st) {
+hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, int mode, hipStream_t st) {
     const int nblocks = a.ntz * a.nty * a.nxc * a.nshots;
 #define STP_FIXED_LAUNCH(RR, VV, TZ, TY, TYR, LX) \
-    if (R == RR && VW == VV) return launch_cfg<RR, VV, TZ, TY, TYR, LX>(a, nblocks, st);
+    if (R == RR && VW == VV) return launch_cfg<RR, VV, TZ, TY, TYR, LX>(a, nblocks, mode, st);
     STP_FD_FIXED(STP_FIXED_LAUNCH)
 #undef STP_FIXED_LAUNCH
     if (R != 4 || VW != 4) return hipErrorInvalidValue;
 #define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL) \
-    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL>(a, nblocks, st);
+    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL>(a, nblocks, mode, st);
     STP_FD_VARIANTS(STP_LAUNCH_CASE)
 #undef STP_LAUNCH_CASE
     return hipErrorInvalidValue;

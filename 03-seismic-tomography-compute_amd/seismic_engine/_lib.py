@@ -78,6 +78,10 @@ SIGNATURES: dict[str, list] = {
     "stp_csr64_add_diagonal": [_i64, _vp, _vp, _vp, _f64, _vp],
     "stp_dot_f64": [_i64, _vp, _vp, C.POINTER(_f64), _vp],
     "stp_csr64_to_f32": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "stp_fd_adjoint": [_vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, C.POINTER(_f64), _vp],
+    "stp_rtm_image_condition_f64": [_i64, _i64, _i64, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "stp_rtm_finalize_f64": [_i64, _vp, _vp, _vp, _vp],
+    "stp_adjoint_gradient_f64": [_i64, _vp, _vp, _vp, _f64, _vp, _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

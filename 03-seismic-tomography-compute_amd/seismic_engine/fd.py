@@ -167,3 +167,55 @@ class FDPropagator:
         if self.prev_damped:
             return self.prev
         return self.prev * self.sponge_factors()
+
+    # ------------------------------------------------------------------ reverse time
+    def reverse_time(self, nt: int, wavelet, *, obs=None, inject=None, snapshot: str = "laplacian",
+                     scale_by_c: bool = True, illumination: bool = False, checkpoint_interval: int = 0,
+                     record: bool = True):
+        """One forward pass from rest (nt steps, checkpointed) and the reverse
+        pass of the adjoint recursion (stp_fd_adjoint).
+
+        obs      : observed traces (nshots, nrcv, nt) -> residual injection and misfit
+        inject   : traces to inject instead (e.g. recorded data for RTM)
+        snapshot : "laplacian" (gradient accumulator) or "field" (RTM image)
+        Returns dict(acc=(nshots,nx,ny,nz), ill=..., traces=(nshots,nrcv,nt), misfit=float|None).
+        The field state (cur/prev) is not modified.
+        """
+        torch = _torch()
+        if not self._have_src:
+            raise RuntimeError("set_sources() before reverse_time()")
+        w = torch.as_tensor(np.asarray(wavelet) if not torch.is_tensor(wavelet) else wavelet,
+                            dtype=torch.float32).to(self.device)
+        if w.dim() == 1:
+            w = w.expand(self.nshots, -1)
+        w = w.contiguous()
+        shp = (self.nshots,) + self.n
+        acc = torch.empty(shp, dtype=torch.float32, device=self.device)
+        ill = torch.empty(shp, dtype=torch.float32, device=self.device) if illumination else None
+        need_tr = record or obs is not None
+        tr = torch.empty((self.nshots, self.nrcv, nt), dtype=torch.float32, device=self.device) \
+            if (need_tr and self.nrcv) else None
+
+        def dev3(a):
+            if a is None:
+                return None
+            t = torch.as_tensor(np.asarray(a) if not torch.is_tensor(a) else a, dtype=torch.float32).to(self.device)
+            return t.reshape(self.nshots, self.nrcv, nt).contiguous()
+
+        o = dev3(obs)
+        inj = dev3(inject)
+        mis = C.c_double(float("nan"))
+        call("stp_fd_adjoint", self._plan, ptr(self.coef), ptr(w), int(w.shape[1]), int(nt), ptr(o), ptr(inj),
+             int(scale_by_c), int(snapshot == "laplacian"), int(checkpoint_interval), ptr(acc), ptr(ill), ptr(tr),
+             C.byref(mis), stream_handle(self.device))
+        return {"acc": acc, "ill": ill, "traces": tr, "misfit": mis.value if obs is not None else None}
+
+    def misfit_gradient(self, nt: int, wavelet, obs, checkpoint_interval: int = 0):
+        """J = 1/2 sum (d - obs)^2 and dJ/dv (nshots summed), FP32 on the GPU.
+        dJ/dv = 2 acc / v with acc = sum_k phi^k L u~^{k-1} (adjoint recursion)."""
+        out = self.reverse_time(nt, wavelet, obs=obs, snapshot="laplacian", scale_by_c=True,
+                                checkpoint_interval=checkpoint_interval, record=True)
+        hw = self.hw
+        v = self.velocity[hw:-hw, hw:-hw, hw:-hw]
+        grad = 2.0 * out["acc"].sum(0) / v
+        return out["misfit"], grad, out["traces"]

@@ -201,6 +201,45 @@ int stp_csr64_to_f32(int64_t rows, int64_t nnz, const int64_t* row_ptr_dev, cons
                      const double* val_dev, int32_t* row_ptr32_dev, int32_t* col32_dev, float* val32_dev,
                      void* stream);
 
+/* ---------------------------------------------------------------- reverse-time pass
+ * RTM image and adjoint-state gradient of the FD propagator (new: the
+ * reference's run_rtm correlates the forward field with itself and its
+ * adjoint_propagate is a time-reversed forward run, wave_equation.cpp:154-194,
+ * 621-688).  One call runs the forward pass from rest with checkpoints every
+ * checkpoint_interval steps (<= 0: ceil(sqrt(nt))), then the reverse pass:
+ * per segment, recompute and snapshot, then step the adjoint recursion
+ *   phi^k = 2 phi^{k+1} - D phi^{k+2} + C L phi^{k+1} + [C(x_r)] e_r^{k-1} delta_r
+ * backwards and accumulate acc_out += phi^k * s^{k-1}, acc_ill += (s^{k-1})^2,
+ * where s^t is the step-t input field u~^t (snap_lap = 0: RTM zero-lag
+ * cross-correlation) or its Laplacian L u~^t (snap_lap = 1: then
+ * dJ/dv = 2 acc_out / v for J = 1/2 sum (d - d_obs)^2 with e = d - d_obs).
+ *   inj_traces_dev : [shot][nrcv][nt] injected at the receivers (may be NULL)
+ *   inj_scale_c    : multiply the injection by C at the receiver cell
+ *   acc_out_dev, acc_ill_dev (may be NULL): [shot][nx*ny*nz], interior-indexed
+ *   traces_out_dev : forward receiver traces [shot][nrcv][nt] (may be NULL)
+ *   obs_dev        : observed traces [shot][nrcv][nt]; when given, the injected
+ *                    traces are the residual e = d - obs of this call's forward
+ *                    pass and *misfit_host = 1/2 sum e^2 (FP64); give either
+ *                    obs_dev or inj_traces_dev
+ * Allocates its checkpoint store (2*ceil(nt/K) + K + 4 fields per shot) for
+ * the duration of the call.  Synchronises `stream`. */
+int stp_fd_adjoint(stp_fd_plan* plan, const float* coef_dev, const float* wavelet_dev, int32_t nw, int32_t nt,
+                   const float* obs_dev, const float* inj_traces_dev, int32_t inj_scale_c, int32_t snap_lap,
+                   int32_t checkpoint_interval, float* acc_out_dev, float* acc_ill_dev, float* traces_out_dev,
+                   double* misfit_host, void* stream);
+
+/* RTMEngine::apply_image_condition (rtm_engine.cpp:24-57): over the interior
+ * nx*ny*nz, f/a := value if finite else 0 (with-halo inputs of strides
+ * ny_stride, nz_stride, halo hw); image += f*a; illum += f*f (interior-indexed). */
+int stp_rtm_image_condition_f64(int64_t nx, int64_t ny, int64_t nz, int32_t hw, int64_t ny_stride, int64_t nz_stride,
+                                const double* fwd_dev, const double* adj_dev, double* image_dev, double* illum_dev,
+                                void* stream);
+/* RTMEngine::finalize_image (rtm_engine.cpp:170-182): refl = image/illum if illum > 1e-10 else 0 */
+int stp_rtm_finalize_f64(int64_t n, const double* image_dev, const double* illum_dev, double* refl_dev, void* stream);
+/* GradientComputer.compute_adjoint_gradient (gradient.py:24-40): g = -fwd*adj*(2 v dt^2) */
+int stp_adjoint_gradient_f64(int64_t n, const double* fwd_dev, const double* adj_dev, const double* vel_dev, double dt,
+                             double* grad_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

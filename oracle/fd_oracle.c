@@ -27,21 +27,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "fd_oracle.h"
+
 #ifndef ORC_REAL
 #define ORC_REAL double
 #endif
 typedef ORC_REAL real;
-
-typedef struct {
-    int64_t nx, ny, nz;      /* interior cells per axis */
-    int32_t hw;              /* halo width */
-    int32_t order;           /* 2 or 8 */
-    double dx, dy, dz, dt;
-    int32_t sponge_width;
-    double sponge_damping;
-    double amplitude_limit;  /* reference default 1e10 */
-    double cfl_safety_factor;/* reference default 0.49 */
-} orc_fd_cfg;
 
 enum { ORC_OK = 0, ORC_NAN_REPAIRED = 1, ORC_CFL_VIOLATION = 2, ORC_FATAL = 3 };
 
@@ -230,12 +221,6 @@ int64_t orc_repair_field_nan(real* d, int64_t count, int64_t nx, int64_t ny, int
         if (!is_valid(d[i])) { d[i] = (real)repl; ++repaired; }
     return repaired;
 }
-
-typedef struct {
-    int64_t total_nan_count;
-    int64_t cells_repaired;
-    int64_t timesteps_recovered;
-} orc_fault_stats;
 
 /* FaultTolerance::attempt_recovery (fault_tolerance.cpp:198-257), 3 attempts */
 static int attempt_recovery(const orc_fd_cfg* c, real* cur, real* prev, int64_t n, orc_fault_stats* st) {
