@@ -61,6 +61,51 @@ def cpu_baseline(threads: int, steps: int = 500):
     return 128**3 * steps / dt / 1e9, dt, cur, tr, v, w, rcv
 
 
+def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: float = 1e-6, reps: int = 3):
+    """Tomography solve: exact straight-ray CSR (seeded source/receiver pairs
+    on opposite faces of a 64^3 model), b = A s_true, CGLS on
+    (A^T A + lam I) x = A^T b to a 1e-6 relative normal-equation residual."""
+    import torch
+    from seismic_engine import synthetic
+    from seismic_engine.sparse import DeviceCSR
+    pts = synthetic.ray_endpoints(nrays, (n, n, n), h=1.0, seed=2026)
+    A = DeviceCSR.from_rays(pts, (n, n, n), 1.0)
+    s_true = torch.as_tensor(1.0 / synthetic.smooth_random_velocity(n, n, n, seed=2027, sigma=20.0 * n / 512 * 4),
+                             dtype=torch.float64, device=dev).reshape(-1)
+    b = A.matvec(s_true)
+    out = {"rows": A.rows, "cols": A.cols, "nnz": A.nnz, "lambda": lam, "tol": tol}
+    # SpMV alone: 8 B/nnz + row offsets + y + gathered x (counted once)
+    x32 = s_true.float()
+    y32 = torch.empty(A.rows, dtype=torch.float32, device=dev)
+    for _ in range(3):
+        y32 = A.matvec(x32)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        y32 = A.matvec(x32)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / 20
+    spmv_bytes = 8 * A.nnz + 4 * (A.rows + 1) + 4 * A.rows + 4 * A.cols
+    out["spmv"] = {"us": round(us, 2), "gb_s": round(spmv_bytes / (us * 1e-6) / 1e9, 1),
+                   "frac_hbm": round(spmv_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    for kind in ("jacobi", "none"):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, info, _ = A.cgls(b, lam, tol, 20000, kind)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if best is None or dt < best[0]:
+                best = (dt, info)
+        dt, info = best
+        out[kind] = {"seconds": round(dt, 5), "iterations": info.iterations, "converged": bool(info.converged),
+                     "true_rel_residual": info.true_rel_residual, "replacements": info.replacements}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +114,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cg", action="store_true", help="skip the tomography-solve timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for PMC byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
@@ -158,9 +204,13 @@ def main() -> int:
                    "step": "one full forward propagation"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fd_step_kernel<4,4,64,16,2,64>", "bytes_per_cell_update": BYTES_PER_CELL,
+                     "kernel": "fd_step_kernel<4,4,128,16,1,64>", "bytes_per_cell_update": BYTES_PER_CELL,
                      "avg_launch_us": round(launch_s * 1e6, 2), "traffic_source": traffic_src},
     }
+
+    if not args.no_cg:
+        log("tomography solve: 200k rays, 64^3, CGLS to 1e-6")
+        result["cg"] = cg_workload(dev)
 
     if not args.no_cpu_baseline:
         del fd
