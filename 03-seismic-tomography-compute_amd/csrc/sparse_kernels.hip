@@ -15,13 +15,49 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "sparse_kernels.h"
 
 namespace stp {
 
 // ------------------------------------------------------------------ SpMV
-template <typename T> __device__ __forceinline__ T shfl_xor_t(T v, int m) { return __shfl_xor(v, m); }
+// Rows are processed by groups of W lanes (RPB = 256 / W rows per block per
+// pass); blocks stride over the rows, so the grid is bounded (<= MAX_BLOCKS)
+// and each block leaves one FP64 partial per reduced quantity.
+constexpr int MAX_BLOCKS = 2048;
+
+template <int W, typename AT, typename XT>
+__device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const int* __restrict__ ci,
+                                      const float* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
+    AT acc = 0;
+    const int b = rp[row], e = rp[row + 1];
+    int k = b + lane;
+    for (; k + 3 * W < e; k += 4 * W) {
+        const float v0 = val[k], v1 = val[k + W], v2 = val[k + 2 * W], v3 = val[k + 3 * W];
+        const int c0 = ci[k], c1 = ci[k + W], c2 = ci[k + 2 * W], c3 = ci[k + 3 * W];
+        const AT x0 = (AT)x[c0], x1 = (AT)x[c1], x2 = (AT)x[c2], x3 = (AT)x[c3];
+        acc += (AT)v0 * x0;
+        acc += (AT)v1 * x1;
+        acc += (AT)v2 * x2;
+        acc += (AT)v3 * x3;
+    }
+    for (; k < e; k += W) acc += (AT)val[k] * (AT)x[ci[k]];
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
+    return acc;
+}
+
+__device__ __forceinline__ void block_partial(double v, double* part) {
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
 
 template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART>
 __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
@@ -30,54 +66,53 @@ __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restri
                                                     double lam, double* __restrict__ part, const int* done) {
     if (done && *done) return;
     constexpr int RPB = 256 / W;
-    const int lane = threadIdx.x % W;
-    const int row = blockIdx.x * RPB + threadIdx.x / W;
-    AT acc = 0;
-    if (row < rows) {
-        const int b = rp[row], e = rp[row + 1];
-        for (int k = b + lane; k < e; k += W) acc += (AT)val[k] * (AT)x[ci[k]];
-    }
-#pragma unroll
-    for (int m = W / 2; m > 0; m >>= 1) acc += shfl_xor_t(acc, m);
-    double out = 0.0;
-    if (row < rows && lane == 0) {
-        double v = alpha * (double)acc;
-        if (FUSE_W) v -= lam * wv[row];
-        y[row] = (YT)v;
-        out = (double)(YT)v;
-    }
-    if (PART) {
-        __shared__ double red[RPB];
-        if (lane == 0) red[threadIdx.x / W] = out * out;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s = 0.0;
-            for (int i = 0; i < RPB; ++i) s += red[i];
-            part[blockIdx.x] = s;
+    const int lane = threadIdx.x % W, sub = threadIdx.x / W;
+    double ps = 0.0;
+    for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
+        const int row = row0 + sub;
+        if (row < rows) {
+            const AT acc = row_dot<W, AT, XT>(rp, ci, val, x, row, lane);
+            if (lane == 0) {
+                double v = alpha * (double)acc;
+                if (FUSE_W) v -= lam * wv[row];
+                const YT yv = (YT)v;
+                y[row] = yv;
+                if (PART) ps += (double)yv * (double)yv;
+            }
         }
     }
+    if (PART) block_partial(ps, part);
 }
 
 template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART>
 static hipError_t spmv_w(int rows, const int* rp, const int* ci, const float* val, const XT* x, YT* y, double alpha,
                          const double* wv, double lam, double* part, const int* done, hipStream_t st) {
-    const int rpb = 256 / W;
-    const int nb = (rows + rpb - 1) / rpb;
-    if (nb == 0) return hipSuccess;
+    const int nb = spmv_blocks(rows, W);
+    if (rows == 0) return hipSuccess;
     hipLaunchKernelGGL((spmv_kernel<W, XT, AT, YT, FUSE_W, PART>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, y,
                        alpha, wv, lam, part, done);
     return hipGetLastError();
 }
 
-int spmv_lanes(double mean_nnz) { return mean_nnz >= 48 ? 64 : (mean_nnz >= 20 ? 32 : 16); }
+int spmv_lanes(double mean_nnz) {
+    if (const char* ev = getenv("STP_SPMV_W")) {
+        const int w = atoi(ev);
+        if (w == 8 || w == 16 || w == 32 || w == 64) return w;
+    }
+    return mean_nnz >= 96 ? 32 : (mean_nnz >= 24 ? 16 : 8);
+}
 
-int spmv_blocks(int rows, int W) { return (rows + 256 / W - 1) / (256 / W); }
+int spmv_blocks(int rows, int W) {
+    const int nb = (rows + 256 / W - 1) / (256 / W);
+    return nb < 1 ? 1 : (nb > MAX_BLOCKS ? MAX_BLOCKS : nb);
+}
 
 #define STP_SPMV_DISPATCH(XT, AT, YT, FW, PT)                                                                   \
     switch (W) {                                                                                                \
         case 64: return spmv_w<64, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
         case 32: return spmv_w<32, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
-        default: return spmv_w<16, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
+        case 16: return spmv_w<16, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
+        default: return spmv_w<8, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st);  \
     }
 
 hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
@@ -109,49 +144,39 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
                                                                double* __restrict__ part_ss, const int* done) {
     if (done && *done) return;
     constexpr int RPB = 256 / W;
-    const int lane = threadIdx.x % W;
-    const int row = blockIdx.x * RPB + threadIdx.x / W;
-    double acc = 0.0;
-    if (row < rows) {
-        const int b = rp[row], e = rp[row + 1];
-        for (int k = b + lane; k < e; k += W) acc += (double)val[k] * r[ci[k]];
-    }
-#pragma unroll
-    for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
-    __shared__ double rsz[RPB], rss[RPB];
-    if (lane == 0) {
-        double sz = 0.0, ss = 0.0;
+    const int lane = threadIdx.x % W, sub = threadIdx.x / W;
+    double psz = 0.0, pss = 0.0;
+    for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
+        const int row = row0 + sub;
         if (row < rows) {
-            const float sv = (float)(acc - lam * x[row]);
-            const float zv = dinv ? dinv[row] * sv : sv;
-            s[row] = sv;
-            z[row] = zv;
-            sz = (double)sv * (double)zv;
-            ss = (double)sv * (double)sv;
+            const double acc = row_dot<W, double, double>(rp, ci, val, r, row, lane);
+            if (lane == 0) {
+                const float sv = (float)(acc - lam * x[row]);
+                const float zv = dinv ? dinv[row] * sv : sv;
+                s[row] = sv;
+                z[row] = zv;
+                psz += (double)sv * (double)zv;
+                pss += (double)sv * (double)sv;
+            }
         }
-        rsz[threadIdx.x / W] = sz;
-        rss[threadIdx.x / W] = ss;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int i = 0; i < RPB; ++i) { a += rsz[i]; b += rss[i]; }
-        part_sz[blockIdx.x] = a;
-        part_ss[blockIdx.x] = b;
-    }
+    block_partial(psz, part_sz);
+    block_partial(pss, part_ss);
 }
 
 hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
                            const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
                            double* part_ss, const int* done, hipStream_t st) {
     const int nb = spmv_blocks(rows, W);
-    if (nb == 0) return hipSuccess;
+    if (rows == 0) return hipSuccess;
     switch (W) {
         case 64: hipLaunchKernelGGL(normal_residual_kernel<64>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
                                     lam, dinv, s, z, part_sz, part_ss, done); break;
         case 32: hipLaunchKernelGGL(normal_residual_kernel<32>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
                                     lam, dinv, s, z, part_sz, part_ss, done); break;
-        default: hipLaunchKernelGGL(normal_residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
+        case 16: hipLaunchKernelGGL(normal_residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
+                                    lam, dinv, s, z, part_sz, part_ss, done); break;
+        default: hipLaunchKernelGGL(normal_residual_kernel<8>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
                                     lam, dinv, s, z, part_sz, part_ss, done); break;
     }
     return hipGetLastError();
@@ -285,26 +310,25 @@ __global__ void __launch_bounds__(256) residual_kernel(int rows, const int* __re
                                                        const float* __restrict__ val, const double* __restrict__ x,
                                                        const double* __restrict__ b, double* __restrict__ r) {
     constexpr int RPB = 256 / W;
-    const int lane = threadIdx.x % W;
-    const int row = blockIdx.x * RPB + threadIdx.x / W;
-    double acc = 0.0;
-    if (row < rows) {
-        const int s = rp[row], e = rp[row + 1];
-        for (int k = s + lane; k < e; k += W) acc += (double)val[k] * x[ci[k]];
+    const int lane = threadIdx.x % W, sub = threadIdx.x / W;
+    for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
+        const int row = row0 + sub;
+        if (row < rows) {
+            const double acc = row_dot<W, double, double>(rp, ci, val, x, row, lane);
+            if (lane == 0) r[row] = b[row] - acc;
+        }
     }
-#pragma unroll
-    for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
-    if (row < rows && lane == 0) r[row] = b[row] - acc;
 }
 
 hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
                     double* r, hipStream_t st) {
     const int nb = spmv_blocks(rows, W);
-    if (nb == 0) return hipSuccess;
+    if (rows == 0) return hipSuccess;
     switch (W) {
         case 64: hipLaunchKernelGGL(residual_kernel<64>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
         case 32: hipLaunchKernelGGL(residual_kernel<32>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
-        default: hipLaunchKernelGGL(residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
+        case 16: hipLaunchKernelGGL(residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
+        default: hipLaunchKernelGGL(residual_kernel<8>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
     }
     return hipGetLastError();
 }
