@@ -1,0 +1,23 @@
+// Device CSR handle shared by the tomography solve and the learned
+// preconditioner (internal; the C ABI sees an opaque pointer).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sparse_kernels.h"
+
+struct stp_csr {
+    int rows = 0, cols = 0;
+    int64_t nnz = 0;
+    int *rp = nullptr, *ci = nullptr, *trp = nullptr, *tci = nullptr;
+    float *val = nullptr, *tval = nullptr;
+    int W = 64, WT = 64;   // lanes per row for A and A^T
+    // workspace
+    double *r = nullptr, *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
+    double *part4 = nullptr, *part5 = nullptr;
+    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr;
+    double *dp = nullptr, *dap = nullptr, *dz = nullptr, *dr = nullptr, *ddinv = nullptr;
+    stp::SolverScalars* sc = nullptr;
+    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0;
+    size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;
+};

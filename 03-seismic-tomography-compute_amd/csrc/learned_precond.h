@@ -1,10 +1,13 @@
-// Learned preconditioner (3-level 3-D convolutional network) used by the
-// flexible CGLS.  Internal interface.
+// Learned preconditioner (3-level 3-D convolutional network) used by CGLS.
+// Internal interface.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 
 namespace stp {
-// z = M(s) for the network held by `handle` (an stp_lpc*); kernels return
+// z = M s for the system prepared in `handle` (an stp_lpc*); kernels return
 // immediately when *done != 0.
 int lpc_apply(void* handle, const float* s, float* z, const int* done, hipStream_t st);
+// number of cells of the prepared system, -1 when none is prepared
+int64_t lpc_cols(const void* handle);
 }  // namespace stp

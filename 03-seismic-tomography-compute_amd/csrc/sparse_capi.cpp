@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "capi_common.h"
+#include "csr_handle.h"
 #include "learned_precond.h"
 #include "seismic_hip.h"
 #include "sparse_kernels.h"
@@ -36,21 +37,6 @@ hipError_t grow(T** p, size_t* cap, size_t n) {
 }
 }  // namespace
 
-struct stp_csr {
-    int rows = 0, cols = 0;
-    int64_t nnz = 0;
-    int *rp = nullptr, *ci = nullptr, *trp = nullptr, *tci = nullptr;
-    float *val = nullptr, *tval = nullptr;
-    int W = 64, WT = 64;   // lanes per row for A and A^T
-    // workspace
-    double *r = nullptr, *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
-    double *part4 = nullptr, *part5 = nullptr;
-    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr;
-    double *dp = nullptr, *dap = nullptr, *dz = nullptr, *dr = nullptr, *ddinv = nullptr;
-    stp::SolverScalars* sc = nullptr;
-    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0;
-    size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;
-};
 
 static void csr_free(stp_csr* h) {
     void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
@@ -246,8 +232,10 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
 
 // ------------------------------------------------------------------ CGLS
 // CGLS on (A^T A + lam I) x = A^T b, x0 = 0.  precond 0: none; 1: Jacobi
-// (1 / column sums of squares + lam); 2: learned network, flexible variant
-// (beta = z.(s - s_prev) / gamma_prev).  Vectors p, q, s, z are FP32 (the SpMV
+// (1 / column sums of squares + lam); 2: learned preconditioner (prepared by
+// stp_lpc_prepare), with the flexible beta = z.(s - s_prev) / gamma_prev,
+// which equals the standard one for a fixed linear M and tolerates its
+// BF16-derived coefficients.  Vectors p, q, s, z are FP32 (the SpMV
 // A p accumulates in FP32); x and r are FP64 and every dot product is FP64.
 // When the recursive residual reaches tol the true residual is recomputed in
 // FP64 (r = b - A x, s = A^T r - lam x); if it is still above tol the
@@ -258,6 +246,8 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     if (!h || !b || !x || max_iter < 0 || lam < 0 || precond < 0 || precond > 2)
         return fail(STP_ERR_INVALID, "stp_cgls: bad argument");
     if (precond == 2 && !precond_handle) return fail(STP_ERR_INVALID, "stp_cgls: learned preconditioner handle missing");
+    if (precond == 2 && stp::lpc_cols(precond_handle) != h->cols)
+        return fail(STP_ERR_STATE, "stp_cgls: learned preconditioner is not prepared for this matrix");
     hipStream_t st = S(stream);
     const int rows = h->rows, cols = h->cols;
     const int nbq = std::max(stp::spmv_blocks(rows, h->W), 1);

@@ -82,6 +82,19 @@ SIGNATURES: dict[str, list] = {
     "stp_rtm_image_condition_f64": [_i64, _i64, _i64, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "stp_rtm_finalize_f64": [_i64, _vp, _vp, _vp, _vp],
     "stp_adjoint_gradient_f64": [_i64, _vp, _vp, _vp, _f64, _vp, _vp],
+    "stp_lpc_create": [C.c_uint64, C.POINTER(_vp)],
+    "stp_lpc_destroy": [_vp],
+    "stp_lpc_num_params": [C.POINTER(_i64)],
+    "stp_lpc_get_params": [_vp, _vp, _i64],
+    "stp_lpc_set_params": [_vp, _vp, _i64],
+    "stp_lpc_features": [_vp, _vp, _vp, _f64, _vp, _vp, _vp],
+    "stp_lpc_prepare": [_vp, _vp, _vp, _f64, _vp],
+    "stp_lpc_coefficients": [_vp, _vp, _vp],
+    "stp_lpc_apply": [_vp, _vp, _vp, _vp],
+    "stp_lpc_loss_grad": [_vp, _vp, _vp, _f64, _vp, _i32, C.POINTER(_f64), _vp, _vp],
+    "stp_lpc_train_step": [_vp, _vp, _vp, _f64, _vp, _i32, _f64, C.POINTER(_f64), _vp],
+    "stp_conv3d_bf16": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _i32, _vp, _vp],
+    "stp_conv3d_bf16_bwd": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

@@ -240,6 +240,50 @@ int stp_rtm_finalize_f64(int64_t n, const double* image_dev, const double* illum
 int stp_adjoint_gradient_f64(int64_t n, const double* fwd_dev, const double* adj_dev, const double* vel_dev, double dt,
                              double* grad_dev, void* stream);
 
+/* ---------------------------------------------------------------- learned preconditioner
+ * New: the reference has only Jacobi-PCG (sparse_ops.cpp:52-113), which stays
+ * the fallback (stp_cgls precond 1).  A 3-level 3-D convolutional network
+ * (16/32/64 channels, 3x3x3, ReLU; BF16 MFMA, FP32 accumulation) reads the
+ * Jacobi-scaled local couplings of N = A^T A + lambda I on an n0 x n1 x n2
+ * cell grid (A's columns, x-major; each dimension a multiple of 4) and
+ * outputs a 27-point stencil G = I + g per cell; the preconditioner is
+ * M = D^-1/2 G G^T D^-1/2 with D = diag(N) (symmetric positive semi-definite).
+ * Training minimises mean_y |G^T J G y - y|^2 over unit probe vectors y,
+ * J = D^-1/2 N D^-1/2.  A handle holds one prepared system at a time. */
+typedef struct stp_lpc stp_lpc;
+int stp_lpc_create(uint64_t seed, stp_lpc** out);
+int stp_lpc_destroy(stp_lpc* h);
+int stp_lpc_num_params(int64_t* n);
+/* FP32 master weights, layers e1, e2, e3, d2, d1, o, each [cout][27][cin] */
+int stp_lpc_get_params(const stp_lpc* h, float* host, int64_t n);
+int stp_lpc_set_params(stp_lpc* h, const float* host, int64_t n);
+/* network input of system A: feat_dev (optional) [cell][32] FP32 copy of the
+ * BF16 features, dsq_dev (optional) [cell] = 1/sqrt(N_ii) */
+int stp_lpc_features(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, float* feat_dev,
+                     float* dsq_dev, void* stream);
+/* features + network: prepares M for A (used by stp_cgls precond 2 and stp_lpc_apply) */
+int stp_lpc_prepare(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, void* stream);
+/* g of the prepared system, [27][cell] */
+int stp_lpc_coefficients(const stp_lpc* h, float* g_dev, void* stream);
+/* z = M s for the prepared system */
+int stp_lpc_apply(const stp_lpc* h, const float* s_dev, float* z_dev, void* stream);
+/* loss and parameter gradient for system A and nprobe probe vectors
+ * probes_dev [nprobe][cell] (unit norm); grad_dev (optional) [num_params] */
+int stp_lpc_loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, const float* probes_dev,
+                      int32_t nprobe, double* loss_host, float* grad_dev, void* stream);
+/* loss_grad followed by one Adam step (beta 0.9 / 0.999) with learning rate lr */
+int stp_lpc_train_step(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, const float* probes_dev,
+                       int32_t nprobe, double lr, double* loss_host, void* stream);
+/* single convolution layer (zero padding), BF16 input [cell][cin] (raw bits),
+ * FP32 weights [cout][27][cin] rounded to BF16, FP32 output [cell][cout];
+ * cin, cout multiples of 16 (cout <= 96, != 80) */
+int stp_conv3d_bf16(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t cout, const uint16_t* in_dev,
+                    const float* w_dev, int32_t relu, float* out_dev, void* stream);
+/* its gradients for dout [cell][cout] (BF16): din [cell][cin] (optional),
+ * dw [cout][27][cin] (optional, overwritten) */
+int stp_conv3d_bf16_bwd(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t cout, const uint16_t* in_dev,
+                        const float* w_dev, const uint16_t* dout_dev, float* din_dev, float* dw_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
