@@ -1,8 +1,12 @@
-"""Build recipe for the gfx950 library (libseismic_hip.so).
+"""Build recipe for the gfx950 library and the C++ drop-in layer.
 
-Compiles every HIP/C++ source under csrc/ with hipcc for gfx950 and links one
-shared library into lib/.  Run directly (`python build.py`) or through
-__graft_entry__.build().
+  lib/libseismic_hip.so  every HIP/C++ source under csrc/ (hipcc, gfx950):
+                         the kernels and the C ABI of include/seismic_hip.h
+  lib/libseismic.so      cpp/*.cpp (g++, host only): the reference's C++
+                         classes (include/seismic/...) over that C ABI
+  lib/dropin_driver      tests/cpp/dropin_driver.cpp, the C++ test program
+
+Run directly (`python build.py`) or through __graft_entry__.build().
 """
 from __future__ import annotations
 
@@ -18,6 +22,13 @@ CSRC = PKG / "csrc"
 LIB = PKG / "lib"
 OBJ = PKG / "_build"
 TARGET = LIB / "libseismic_hip.so"
+CPP = PKG / "cpp"
+HOST_TARGET = LIB / "libseismic.so"
+DRIVER_SRC = ROOT / "tests" / "cpp" / "dropin_driver.cpp"
+DRIVER = LIB / "dropin_driver"
+GXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
+              f"-I{ROOT / 'include' / 'seismic'}"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("STP_OFFLOAD_ARCH", "gfx950")
@@ -56,9 +67,31 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, TARGET)
+    build_host()
     if verbose:
-        print(f"built {TARGET.relative_to(ROOT)} from {len(srcs)} sources", flush=True)
+        print(f"built {TARGET.relative_to(ROOT)} from {len(srcs)} sources, {HOST_TARGET.name} and "
+              f"{DRIVER.name}", flush=True)
     return TARGET
+
+
+def _run(cmd: list[str], what: str) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_host() -> None:
+    """The C++ drop-in layer (host code only, linked against the HIP library)."""
+    srcs = sorted(str(p) for p in CPP.glob("*.cpp"))
+    link = [f"-L{LIB}", "-lseismic_hip", "-Wl,-rpath,$ORIGIN"]
+    tmp = HOST_TARGET.with_suffix(".so.tmp")
+    _run([GXX, *HOST_FLAGS, "-shared", *srcs, *link, "-o", str(tmp)], "host library")
+    os.replace(tmp, HOST_TARGET)
+    if DRIVER_SRC.exists():
+        tmp = DRIVER.with_name(DRIVER.name + ".tmp")
+        _run([GXX, *HOST_FLAGS, str(DRIVER_SRC), f"-L{LIB}", "-lseismic", "-lseismic_hip", "-Wl,-rpath,$ORIGIN",
+              "-o", str(tmp)], "drop-in driver")
+        os.replace(tmp, DRIVER)
 
 
 if __name__ == "__main__":

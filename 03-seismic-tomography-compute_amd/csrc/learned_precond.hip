@@ -5,22 +5,30 @@
 // Jacobi-scaled couplings of N to its 26 neighbours,
 //   J[i][i+d] = N[i][i+d] / sqrt(N[i][i] N[i+d][i+d]),
 // plus a constant channel (which marks the model boundary through the zero
-// padding), and it outputs 27 coefficients per cell of a local operator
-// G = I + g (3x3x3 stencil).  The preconditioner is
+// padding), and it outputs the 63 coefficients of row i of an upper-triangular
+// factor G (taps d in [-2,2]^3 lexicographically >= 0, the centre first; the
+// diagonal is exp(output) > 0).  The preconditioner is
 //   M = D^-1/2 G G^T D^-1/2,   D = diag(N),
-// which is linear, symmetric and positive semi-definite for any weights, so
-// the CG theory still holds; the network runs once per system and each
-// iteration costs two 27-point stencils.  Training minimises
-//   sum_y |G^T J G y - y|^2 / |y|^2     (J = D^-1/2 N D^-1/2)
-// over probe vectors y (scaled CG residuals and noise): the loss is zero when
-// G G^T = J^-1.
+// symmetric positive definite for any weights, so the CG theory holds; the
+// network runs once per system and each iteration costs two 63-tap stencils.
+//
+// Training minimises Kaporin's condition number of G^T J G
+// (J = D^-1/2 N D^-1/2):
+//   K = (tr(G^T J G) / n) / det(G^T J G)^(1/n),
+//   log K = log(tr(G^T J G) / n) - (2/n) sum_i log G_ii - (1/n) log det J,
+// with the trace estimated from Rademacher probe vectors y, E[y y^T] = I:
+//   tr(G^T J G) ~ (1/P) sum_y (G y)^T J (G y).
+// K >= 1 with equality iff G G^T is a multiple of J^-1, it bounds the CG
+// iteration count (Kaporin 1994), and it is invariant to the scale of G.  Its
+// minimiser over a fixed pattern is the factorised sparse approximate inverse
+// (FSAI) of J; the network predicts such a factor from local couplings.
 //
 // Network: 3 levels, 16/32/64 channels, 3x3x3 convolutions, ReLU, average
 // pooling down, nearest upsampling and skip concatenation up (conv3d.hip),
 // BF16 weights and activations, FP32 accumulation and FP32 master weights
 // (Adam).  Layers, [cout][27][cin]:
 //   e1 32->16, e2 16->32 (after pool), e3 32->64 (after pool),
-//   d2 (64+32)->32, d1 (32+16)->16, o 16->32 (27 used, no ReLU).
+//   d2 (64+32)->32, d1 (32+16)->16, o 16->64 (63 used, no ReLU).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,13 +50,16 @@ using stp::S;
 namespace {
 
 constexpr int NFEAT = 32;   // input channels (26 couplings + constant + padding)
-constexpr int NOUT = 32;    // output channels (27 stencil coefficients + padding)
-constexpr int NTAP = 27;
+constexpr int NTAP = 27;    // 3x3x3 couplings and convolution taps
+constexpr int RG = 2;       // stencil radius of the factor G
+constexpr int NG = 63;      // taps of G: offsets in [-2,2]^3 lexicographically >= 0
+constexpr int NOUT = 64;    // network output channels (NG used)
+constexpr int PARTS = 1024; // FP64 block partial sums per reduction
 
 struct LayerDef {
     int cin, cout;
 };
-constexpr LayerDef LAYERS[6] = {{32, 16}, {16, 32}, {32, 64}, {96, 32}, {48, 16}, {16, 32}};
+constexpr LayerDef LAYERS[6] = {{32, 16}, {16, 32}, {32, 64}, {96, 32}, {48, 16}, {16, NOUT}};
 enum { E1 = 0, E2, E3, D2, D1, LO };
 
 int64_t layer_size(int l) { return (int64_t)LAYERS[l].cin * LAYERS[l].cout * NTAP; }
@@ -132,68 +143,101 @@ __global__ void features_kernel(int n0, int n1, int n2, const double* nloc, doub
     }
 }
 
-// ------------------------------------------------------------------ stencils of G = I + g
-// u = sy .* (G (sx .* y))   (sx, sy may be null = 1)
-__global__ void apply_g_kernel(int n0, int n1, int n2, const float* __restrict__ g, const float* __restrict__ y,
-                               const float* __restrict__ sx, const float* __restrict__ sy, float* __restrict__ u,
-                               const int* done) {
+// ------------------------------------------------------------------ stencils of G
+// Tap k of G couples cell i to cell i + d_k, d_k = the (62 + k)-th offset of
+// [-2,2]^3 in lexicographic (x, y, z) order; k = 0 is the diagonal.
+__device__ __forceinline__ void g_offset(int k, int& dx, int& dy, int& dz) {
+    const int l = 62 + k;
+    dx = l / 25 - RG;
+    dy = (l / 5) % 5 - RG;
+    dz = l % 5 - RG;
+}
+
+// u = sy .* (G (sx .* y)):  u_i = sum_k g[k][i] x_{i + d_k}   (sx, sy may be null = 1)
+__global__ void __launch_bounds__(256) apply_g_kernel(int n0, int n1, int n2, const float* __restrict__ g,
+                                                      const float* __restrict__ y, const float* __restrict__ sx,
+                                                      const float* __restrict__ sy, float* __restrict__ u,
+                                                      const int* done) {
     if (done && *done) return;
     const int64_t V = (int64_t)n0 * n1 * n2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
         const int z = (int)(i % n2), yv = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
-        float acc = y[i] * (sx ? sx[i] : 1.f);
+        float acc = 0.f;
 #pragma unroll
-        for (int s = 0; s < NTAP; ++s) {
-            const int dx = s / 9 - 1, dy = (s / 3) % 3 - 1, dz = s % 3 - 1;
+        for (int k = 0; k < NG; ++k) {
+            int dx, dy, dz;
+            g_offset(k, dx, dy, dz);
             const int xx = x + dx, yy = yv + dy, zz = z + dz;
             if ((unsigned)xx < (unsigned)n0 && (unsigned)yy < (unsigned)n1 && (unsigned)zz < (unsigned)n2) {
                 const int64_t j = ((int64_t)xx * n1 + yy) * n2 + zz;
-                acc += g[s * V + i] * y[j] * (sx ? sx[j] : 1.f);
+                acc += g[k * V + i] * y[j] * (sx ? sx[j] : 1.f);
             }
         }
         u[i] = sy ? sy[i] * acc : acc;
     }
 }
 
-// w = sy .* (G^T (sx .* v)):  w_j = sum_s g[s][j - d_s] x[j - d_s] + x_j
-__global__ void apply_gt_kernel(int n0, int n1, int n2, const float* __restrict__ g, const float* __restrict__ v,
-                                const float* __restrict__ sx, const float* __restrict__ sy, float* __restrict__ w,
-                                const int* done) {
+// w = sy .* (G^T (sx .* v)):  w_j = sum_k g[k][j - d_k] x_{j - d_k}
+__global__ void __launch_bounds__(256) apply_gt_kernel(int n0, int n1, int n2, const float* __restrict__ g,
+                                                       const float* __restrict__ v, const float* __restrict__ sx,
+                                                       const float* __restrict__ sy, float* __restrict__ w,
+                                                       const int* done) {
     if (done && *done) return;
     const int64_t V = (int64_t)n0 * n1 * n2;
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < V; j += (int64_t)gridDim.x * blockDim.x) {
         const int z = (int)(j % n2), yv = (int)((j / n2) % n1), x = (int)(j / ((int64_t)n1 * n2));
-        float acc = v[j] * (sx ? sx[j] : 1.f);
+        float acc = 0.f;
 #pragma unroll
-        for (int s = 0; s < NTAP; ++s) {
-            const int dx = s / 9 - 1, dy = (s / 3) % 3 - 1, dz = s % 3 - 1;
+        for (int k = 0; k < NG; ++k) {
+            int dx, dy, dz;
+            g_offset(k, dx, dy, dz);
             const int xx = x - dx, yy = yv - dy, zz = z - dz;
             if ((unsigned)xx < (unsigned)n0 && (unsigned)yy < (unsigned)n1 && (unsigned)zz < (unsigned)n2) {
                 const int64_t i = ((int64_t)xx * n1 + yy) * n2 + zz;
-                acc += g[s * V + i] * v[i] * (sx ? sx[i] : 1.f);
+                acc += g[k * V + i] * v[i] * (sx ? sx[i] : 1.f);
             }
         }
         w[j] = sy ? sy[j] * acc : acc;
     }
 }
 
-// dg[s][i] += a_i b_{i+d_s} + c_i e_{i+d_s}
-__global__ void coef_grad_kernel(int n0, int n1, int n2, const float* a, const float* b, const float* c,
-                                 const float* e, float* dg) {
+// dg[k][i] += a_i b_{i + d_k}
+__global__ void __launch_bounds__(256) coef_grad_kernel(int n0, int n1, int n2, const float* __restrict__ a,
+                                                        const float* __restrict__ b, float* __restrict__ dg) {
     const int64_t V = (int64_t)n0 * n1 * n2;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
         const int z = (int)(i % n2), yv = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
-        const float ai = a[i], ci = c[i];
+        const float ai = a[i];
 #pragma unroll
-        for (int s = 0; s < NTAP; ++s) {
-            const int dx = s / 9 - 1, dy = (s / 3) % 3 - 1, dz = s % 3 - 1;
+        for (int k = 0; k < NG; ++k) {
+            int dx, dy, dz;
+            g_offset(k, dx, dy, dz);
             const int xx = x + dx, yy = yv + dy, zz = z + dz;
             if ((unsigned)xx < (unsigned)n0 && (unsigned)yy < (unsigned)n1 && (unsigned)zz < (unsigned)n2) {
                 const int64_t j = ((int64_t)xx * n1 + yy) * n2 + zz;
-                dg[s * V + i] += ai * b[j] + ci * e[j];
+                dg[k * V + i] += ai * b[j];
             }
         }
     }
+}
+
+// network output (SoA, NOUT rows) -> coefficients: diagonal exp(o), clamped
+// to exp(+-30); FP64 partial sums of the clamped exponent (= log G_ii)
+__global__ void finalize_g_kernel(int64_t V, float* g, double* part) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
+        const float o = fminf(fmaxf(g[i], -30.f), 30.f);
+        g[i] = expf(o);
+        acc += o;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part) part[blockIdx.x] = red[0];
 }
 
 // out = x .* s ; out2 = s .* (t + lam * x .* s)   (two helpers of J)
@@ -206,14 +250,11 @@ __global__ void jfinish_kernel(int64_t n, const float* t, const float* xs, const
         out[i] = s[i] * (t[i] + lam * xs[i]);
 }
 
-// rho = k (w - y); FP64 partial sums of (w - y)^2
-__global__ void residual_loss_kernel(int64_t n, const float* w, const float* y, float k, float* rho, double* part) {
+// FP64 partial sums of a . b (one per block of 256)
+__global__ void dot_part_kernel(int64_t n, const float* a, const float* b, double* part) {
     double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const float r = w[i] - y[i];
-        rho[i] = k * r;
-        acc += (double)r * r;
-    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += (double)a[i] * (double)b[i];
     __shared__ double red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -224,13 +265,20 @@ __global__ void residual_loss_kernel(int64_t n, const float* w, const float* y, 
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// dO[p][c] = bf16(dg[c][p]) for c < 27, 0 above
-__global__ void soa_to_cl_kernel(int64_t V, const float* dg, bf16_t* d) {
+// gradient of log K with respect to the network output, channels-last BF16:
+//   channel k >= 1: coef * dg[k][p];  channel 0: coef * dg[0][p] * G_pp - 2/V;
+//   channels >= NG: 0
+__global__ void out_grad_kernel(int64_t V, const float* dg, const float* g, float coef, float diag, bf16_t* d) {
     const int64_t n = V * NOUT;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const int c = (int)(k % NOUT);
         const int64_t p = k / NOUT;
-        d[k] = f2bf(c < NTAP ? dg[c * V + p] : 0.f);
+        float v = 0.f;
+        if (c == 0)
+            v = coef * dg[p] * g[p] - diag;
+        else if (c < NG)
+            v = coef * dg[c * V + p];
+        d[k] = f2bf(v);
     }
 }
 
@@ -267,9 +315,8 @@ struct stp_lpc {
     float *dg = nullptr, *de = nullptr, *dcat1 = nullptr, *dd = nullptr, *da = nullptr, *dcat2 = nullptr,
           *dc = nullptr, *db = nullptr, *dpb = nullptr, *dpa = nullptr;
     bf16_t *dO = nullptr, *dpre = nullptr;
-    float *u = nullptr, *w1 = nullptr, *t = nullptr, *v = nullptr, *w = nullptr, *rho = nullptr, *dv = nullptr,
-          *du = nullptr, *q = nullptr;
-    double* part = nullptr;
+    float *u = nullptr, *w1 = nullptr, *t = nullptr, *v = nullptr, *q = nullptr;
+    double* part = nullptr;   // 2 x PARTS FP64 block partial sums
     int64_t qcap = 0;
     // state of the prepared preconditioner
     bool prepared = false;
@@ -280,8 +327,7 @@ namespace {
 void free_grid(stp_lpc* h) {
     void* ps[] = {h->feat, h->a,  h->pa,    h->b,  h->pb, h->c,  h->cat2, h->d,   h->cat1, h->e,  h->g,
                   h->dsq,  h->tmp, h->nloc, h->dg, h->de, h->dcat1, h->dd, h->da, h->dcat2, h->dc, h->db,
-                  h->dpb,  h->dpa, h->dO,   h->dpre, h->u, h->w1, h->t,  h->v,  h->w,    h->rho, h->dv, h->du,
-                  h->part};
+                  h->dpb,  h->dpa, h->dO,   h->dpre, h->u, h->w1, h->t,  h->v,  h->part};
     for (void* p : ps)
         if (p) hipFree(p);
     h->feat = h->a = h->pa = h->b = h->pb = h->c = h->cat2 = h->d = h->cat1 = h->e = nullptr;
@@ -289,7 +335,7 @@ void free_grid(stp_lpc* h) {
     h->nloc = nullptr;
     h->dg = h->de = h->dcat1 = h->dd = h->da = h->dcat2 = h->dc = h->db = h->dpb = h->dpa = nullptr;
     h->dO = h->dpre = nullptr;
-    h->u = h->w1 = h->t = h->v = h->w = h->rho = h->dv = h->du = nullptr;
+    h->u = h->w1 = h->t = h->v = nullptr;
     h->part = nullptr;
     h->V = 0;
     h->n0 = h->n1 = h->n2 = 0;
@@ -328,11 +374,12 @@ int ensure_grid(stp_lpc* h, const int64_t* n, bool train) {
     STP_HIP(dalloc(&h->cat1, V * 48));
     STP_HIP(dalloc(&h->e, V * 16));
     STP_HIP(dalloc(&h->g, V * NOUT));
+    STP_HIP(dalloc(&h->part, 2 * PARTS));
     STP_HIP(dalloc(&h->dsq, V));
     STP_HIP(dalloc(&h->tmp, V));
     STP_HIP(dalloc(&h->nloc, V * NTAP));
     if (keep_train) {
-        STP_HIP(dalloc(&h->dg, V * NOUT));
+        STP_HIP(dalloc(&h->dg, V * NG));
         STP_HIP(dalloc(&h->de, V * 16));
         STP_HIP(dalloc(&h->dcat1, V * 48));
         STP_HIP(dalloc(&h->dd, V2 * 32));
@@ -348,11 +395,6 @@ int ensure_grid(stp_lpc* h, const int64_t* n, bool train) {
         STP_HIP(dalloc(&h->w1, V));
         STP_HIP(dalloc(&h->t, V));
         STP_HIP(dalloc(&h->v, V));
-        STP_HIP(dalloc(&h->w, V));
-        STP_HIP(dalloc(&h->rho, V));
-        STP_HIP(dalloc(&h->dv, V));
-        STP_HIP(dalloc(&h->du, V));
-        STP_HIP(dalloc(&h->part, 1024));
         h->train_ws = true;
     }
     return STP_OK;
@@ -397,6 +439,9 @@ int net_forward(stp_lpc* h, hipStream_t st) {
     STP_HIP(upcat_fwd(a0, a1, a2, 32, 16, h->d, h->a, h->cat1, st));
     STP_HIP(conv3d_fwd(shp(a0, a1, a2, D1), h->cat1, W(D1), 1, CONV_OUT_BF16, h->e, 16, 0, st));
     STP_HIP(conv3d_fwd(shp(a0, a1, a2, LO), h->e, W(LO), 0, CONV_OUT_F32_SOA, h->g, 0, 0, st));
+    // diagonal = exp(output 0); partial sums of log G_ii into part[PARTS ..]
+    hipLaunchKernelGGL(finalize_g_kernel, dim3(PARTS), dim3(256), 0, st, h->V, h->g, h->part + PARTS);
+    STP_HIP(hipGetLastError());
     return STP_OK;
 }
 
@@ -409,7 +454,7 @@ int conv_bwd_data(stp_lpc* h, int l, int n0, int n1, int n2, const bf16_t* dpre,
     return STP_OK;
 }
 
-// parameter gradient (into h->grad, overwritten) from dL/dg in h->dg (SoA, rows 27.. zero)
+// parameter gradient (into h->grad, overwritten) from dL/d(output) in h->dO
 int net_backward(stp_lpc* h, hipStream_t st) {
     using namespace stp;
     const int a0 = h->n0, a1 = h->n1, a2 = h->n2;
@@ -419,8 +464,6 @@ int net_backward(stp_lpc* h, hipStream_t st) {
     auto G = [&](int l) { return h->grad + layer_off(l); };
     int rc;
     STP_HIP(hipMemsetAsync(h->grad, 0, sizeof(float) * total_params(), st));
-    hipLaunchKernelGGL(soa_to_cl_kernel, dim3(gridn(V * NOUT)), dim3(256), 0, st, V, h->dg, h->dO);
-    STP_HIP(hipGetLastError());
     // o
     STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, LO), h->e, h->dO, G(LO), st));
     if ((rc = conv_bwd_data(h, LO, a0, a1, a2, h->dO, h->de, st)) != STP_OK) return rc;
@@ -462,7 +505,11 @@ int apply_j(stp_lpc* h, const stp_csr* A, double lam, const float* x, float* out
     return STP_OK;
 }
 
-// features + forward + loss + backward (gradient in h->grad); loss averaged over probes
+// features + forward + loss + backward (gradient in h->grad).  Loss:
+//   log K~ = log(q / (P V)) - (2/V) sum_i log G_ii,   q = sum_y (G y)^T J (G y)
+// (the constant -(1/V) log det J is left out); dlogK/dg[k][i] =
+// (2/q) sum_y (J G y)_i y_{i+d_k}, and through G_ii = exp(o_i) the diagonal
+// output gets (2/q) sum_y (J G y)_i y_i G_ii - 2/V.
 int loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, const float* probes, int nprobe,
               double* loss, hipStream_t st) {
     int rc = check_system(A, n);
@@ -481,34 +528,35 @@ int loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, const 
     if ((rc = net_forward(h, st)) != STP_OK) return rc;
     const int64_t V = h->V;
     const int n0 = h->n0, n1 = h->n1, n2 = h->n2;
-    STP_HIP(hipMemsetAsync(h->dg, 0, sizeof(float) * V * NOUT, st));
-    const unsigned nbl = std::min<unsigned>(1024, gridn(V));
-    std::vector<double> parts((size_t)nprobe * nbl);
+    STP_HIP(hipMemsetAsync(h->dg, 0, sizeof(float) * V * NG, st));
+    std::vector<double> parts((size_t)(nprobe + 1) * PARTS);
     for (int k = 0; k < nprobe; ++k) {
         const float* y = probes + (int64_t)k * V;
         hipLaunchKernelGGL(apply_g_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->g, y, nullptr, nullptr,
                            h->u, nullptr);
+        STP_HIP(hipGetLastError());
         if ((rc = apply_j(h, A, lam, h->u, h->v, st)) != STP_OK) return rc;
-        hipLaunchKernelGGL(apply_gt_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->g, h->v, nullptr, nullptr,
-                           h->w, nullptr);
-        hipLaunchKernelGGL(residual_loss_kernel, dim3(nbl), dim3(256), 0, st, V, h->w, y, 2.0f / nprobe, h->rho,
-                           h->part);
+        hipLaunchKernelGGL(dot_part_kernel, dim3(PARTS), dim3(256), 0, st, V, h->u, h->v, h->part);
+        hipLaunchKernelGGL(coef_grad_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->v, y, h->dg);
         STP_HIP(hipGetLastError());
-        STP_HIP(hipMemcpyAsync(parts.data() + (size_t)k * nbl, h->part, sizeof(double) * nbl, hipMemcpyDeviceToHost,
-                               st));
-        // dL/dv = G rho ; dL/du = J dL/dv ; dg += v (x) rho + du (x) y
-        hipLaunchKernelGGL(apply_g_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->g, h->rho, nullptr,
-                           nullptr, h->dv, nullptr);
-        if ((rc = apply_j(h, A, lam, h->dv, h->du, st)) != STP_OK) return rc;
-        hipLaunchKernelGGL(coef_grad_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->v, h->rho, h->du, y,
-                           h->dg);
-        STP_HIP(hipGetLastError());
+        STP_HIP(hipMemcpyAsync(parts.data() + (size_t)k * PARTS, h->part, sizeof(double) * PARTS,
+                               hipMemcpyDeviceToHost, st));
     }
+    STP_HIP(hipMemcpyAsync(parts.data() + (size_t)nprobe * PARTS, h->part + PARTS, sizeof(double) * PARTS,
+                           hipMemcpyDeviceToHost, st));
+    STP_HIP(hipStreamSynchronize(st));
+    double q = 0.0, logdiag = 0.0;
+    for (int k = 0; k < nprobe; ++k)
+        for (int j = 0; j < PARTS; ++j) q += parts[(size_t)k * PARTS + j];
+    for (int j = 0; j < PARTS; ++j) logdiag += parts[(size_t)nprobe * PARTS + j];
+    if (!(q > 0.0) || !std::isfinite(q))
+        return fail(STP_ERR_INVALID, "learned preconditioner: non-positive or non-finite trace estimate");
+    if (loss) *loss = std::log(q / ((double)nprobe * (double)V)) - 2.0 * logdiag / (double)V;
+    hipLaunchKernelGGL(out_grad_kernel, dim3(gridn(V * NOUT)), dim3(256), 0, st, V, h->dg, h->g, (float)(2.0 / q),
+                       (float)(2.0 / (double)V), h->dO);
+    STP_HIP(hipGetLastError());
     if ((rc = net_backward(h, st)) != STP_OK) return rc;
     STP_HIP(hipStreamSynchronize(st));
-    double l = 0.0;
-    for (double p : parts) l += p;
-    if (loss) *loss = l / nprobe;
     return STP_OK;
 }
 
@@ -638,7 +686,7 @@ int stp_lpc_prepare(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, 
 int stp_lpc_coefficients(const stp_lpc* h, float* g_dev, void* stream) {
     if (!h || !g_dev) return fail(STP_ERR_INVALID, "stp_lpc_coefficients: null argument");
     if (!h->prepared) return fail(STP_ERR_STATE, "stp_lpc_coefficients: not prepared for a system");
-    STP_HIP(hipMemcpyAsync(g_dev, h->g, sizeof(float) * NTAP * h->V, hipMemcpyDeviceToDevice, S(stream)));
+    STP_HIP(hipMemcpyAsync(g_dev, h->g, sizeof(float) * NG * h->V, hipMemcpyDeviceToDevice, S(stream)));
     return STP_OK;
 }
 

@@ -43,6 +43,11 @@ __global__ void adjoint_gradient_kernel(int64_t n, const double* f, const double
     }
 }
 
+__global__ void accumulate_kernel(int64_t n, const float* x, double* y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = y[i] + (double)x[i];
+}
+
 unsigned grid(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 8192) b = 8192;
@@ -86,6 +91,14 @@ int stp_adjoint_gradient_f64(int64_t n, const double* fwd, const double* adj, co
     if (n == 0) return STP_OK;
     hipLaunchKernelGGL(stp::adjoint_gradient_kernel, dim3(stp::grid(n)), dim3(256), 0, S(stream), n, fwd, adj, vel,
                        dt, grad);
+    STP_HIP(hipGetLastError());
+    return STP_OK;
+}
+
+int stp_accumulate_f32_f64(int64_t n, const float* x, double* y, void* stream) {
+    if (n < 0 || (n > 0 && (!x || !y))) return fail(STP_ERR_INVALID, "stp_accumulate_f32_f64: bad argument");
+    if (n == 0) return STP_OK;
+    hipLaunchKernelGGL(stp::accumulate_kernel, dim3(stp::grid(n)), dim3(256), 0, S(stream), n, x, y);
     STP_HIP(hipGetLastError());
     return STP_OK;
 }

@@ -81,6 +81,8 @@ SIGNATURES: dict[str, list] = {
     "stp_fd_adjoint": [_vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, C.POINTER(_f64), _vp],
     "stp_rtm_image_condition_f64": [_i64, _i64, _i64, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     "stp_rtm_finalize_f64": [_i64, _vp, _vp, _vp, _vp],
+    "stp_accumulate_f32_f64": [_i64, _vp, _vp, _vp],
+    "stp_smooth_gradient_f64": [_i64, _i64, _i64, _f64, _vp, _vp, _vp, _vp],
     "stp_adjoint_gradient_f64": [_i64, _vp, _vp, _vp, _f64, _vp, _vp],
     "stp_lpc_create": [C.c_uint64, C.POINTER(_vp)],
     "stp_lpc_destroy": [_vp],

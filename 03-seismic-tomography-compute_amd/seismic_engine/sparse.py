@@ -271,7 +271,30 @@ class SparseOps:
         return _out(x, b)
 
     @staticmethod
+    def smooth_gradient(gradient, nx: int, ny: int, nz: int, sigma: float):
+        """Gaussian smoothing with radius int(3 sigma) and in-range
+        renormalisation (sparse_ops.cpp:115-158), FP64 on the GPU as three
+        separable passes; a gradient shorter than nx*ny*nz gives zeros."""
+        torch = _torch()
+        n = int(nx) * int(ny) * int(nz)
+        g = _as_dev(gradient, torch.float64)
+        if g.numel() < n:
+            return _out(torch.zeros(n, dtype=torch.float64, device=g.device), gradient)
+        g = g[:n].contiguous()
+        out = torch.empty_like(g)
+        work = torch.empty_like(g)
+        call("stp_smooth_gradient_f64", C.c_int64(nx), C.c_int64(ny), C.c_int64(nz), C.c_double(sigma), ptr(g),
+             ptr(out), ptr(work), stream_handle())
+        return _out(out, gradient)
+
+    @staticmethod
+    def l2_norm(a) -> float:
+        return float(np.sqrt(SparseOps.dot_product(a, a)))
+
+    @staticmethod
     def dot_product(a, b) -> float:
+        if isinstance(a, CSRMatrix) or isinstance(b, CSRMatrix):
+            raise TypeError("dot_product(a, b) takes two 1-D arrays")
         torch = _torch()
         av = _as_dev(a, torch.float64)
         bv = _as_dev(b, torch.float64)

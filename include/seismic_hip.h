@@ -196,6 +196,12 @@ int stp_csr64_add_diagonal(int64_t rows, const int64_t* row_ptr_dev, const int64
                            double alpha, void* stream);
 /* FP64 dot product (SparseOps::dot_product, sparse_ops.cpp:172-187; tree order) */
 int stp_dot_f64(int64_t n, const double* a_dev, const double* b_dev, double* out_host, void* stream);
+/* SparseOps::smooth_gradient (sparse_ops.cpp:115-158): Gaussian weights
+ * exp(-|d|^2/(2 sigma^2)) over |d_x|,|d_y|,|d_z| <= (int)(3 sigma) (<= 64),
+ * renormalised by the in-range weight sum; in/out/work are nx*ny*nz FP64
+ * (x-major), work is scratch.  out must not alias in.  Asynchronous. */
+int stp_smooth_gradient_f64(int64_t nx, int64_t ny, int64_t nz, double sigma, const double* in_dev, double* out_dev,
+                            double* work_dev, void* stream);
 /* FP64/int64 CSR -> FP32/int32 arrays for stp_csr_create */
 int stp_csr64_to_f32(int64_t rows, int64_t nnz, const int64_t* row_ptr_dev, const int64_t* col_dev,
                      const double* val_dev, int32_t* row_ptr32_dev, int32_t* col32_dev, float* val32_dev,
@@ -236,6 +242,9 @@ int stp_rtm_image_condition_f64(int64_t nx, int64_t ny, int64_t nz, int32_t hw, 
                                 void* stream);
 /* RTMEngine::finalize_image (rtm_engine.cpp:170-182): refl = image/illum if illum > 1e-10 else 0 */
 int stp_rtm_finalize_f64(int64_t n, const double* image_dev, const double* illum_dev, double* refl_dev, void* stream);
+/* y += x elementwise, FP32 into FP64 (adds a reverse-time pass's FP32 image or
+ * illumination into an FP64 RTMEngine accumulator; new) */
+int stp_accumulate_f32_f64(int64_t n, const float* x_dev, double* y_dev, void* stream);
 /* GradientComputer.compute_adjoint_gradient (gradient.py:24-40): g = -fwd*adj*(2 v dt^2) */
 int stp_adjoint_gradient_f64(int64_t n, const double* fwd_dev, const double* adj_dev, const double* vel_dev, double dt,
                              double* grad_dev, void* stream);
@@ -244,12 +253,17 @@ int stp_adjoint_gradient_f64(int64_t n, const double* fwd_dev, const double* adj
  * New: the reference has only Jacobi-PCG (sparse_ops.cpp:52-113), which stays
  * the fallback (stp_cgls precond 1).  A 3-level 3-D convolutional network
  * (16/32/64 channels, 3x3x3, ReLU; BF16 MFMA, FP32 accumulation) reads the
- * Jacobi-scaled local couplings of N = A^T A + lambda I on an n0 x n1 x n2
- * cell grid (A's columns, x-major; each dimension a multiple of 4) and
- * outputs a 27-point stencil G = I + g per cell; the preconditioner is
- * M = D^-1/2 G G^T D^-1/2 with D = diag(N) (symmetric positive semi-definite).
- * Training minimises mean_y |G^T J G y - y|^2 over unit probe vectors y,
- * J = D^-1/2 N D^-1/2.  A handle holds one prepared system at a time. */
+ * Jacobi-scaled couplings of N = A^T A + lambda I to the 26 neighbours of each
+ * cell of an n0 x n1 x n2 grid (A's columns, x-major; each dimension a
+ * multiple of 4) and outputs row i of an upper-triangular factor G: 63 taps,
+ * offsets in [-2,2]^3 lexicographically >= (0,0,0), the diagonal first and
+ * positive (exp of the network output).  The preconditioner is
+ * M = D^-1/2 G G^T D^-1/2 with D = diag(N): symmetric positive definite for
+ * any weights.  Training minimises Kaporin's condition number
+ * log K = log(tr(G^T J G)/n) - (2/n) sum log G_ii (+ const) of
+ * J = D^-1/2 N D^-1/2, the trace estimated from probe vectors with
+ * E[y y^T] = I (Rademacher +-1).  Untrained (zero output layer), G = I and
+ * M is Jacobi.  A handle holds one prepared system at a time. */
 typedef struct stp_lpc stp_lpc;
 int stp_lpc_create(uint64_t seed, stp_lpc** out);
 int stp_lpc_destroy(stp_lpc* h);
@@ -263,12 +277,15 @@ int stp_lpc_features(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double
                      float* dsq_dev, void* stream);
 /* features + network: prepares M for A (used by stp_cgls precond 2 and stp_lpc_apply) */
 int stp_lpc_prepare(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, void* stream);
-/* g of the prepared system, [27][cell] */
+/* coefficients of G for the prepared system, [63][cell]: tap k couples cell
+ * i to cell i + d_k, d_k the (62 + k)-th offset of [-2,2]^3 in lexicographic
+ * (x, y, z) order (k = 0: the diagonal) */
 int stp_lpc_coefficients(const stp_lpc* h, float* g_dev, void* stream);
 /* z = M s for the prepared system */
 int stp_lpc_apply(const stp_lpc* h, const float* s_dev, float* z_dev, void* stream);
-/* loss and parameter gradient for system A and nprobe probe vectors
- * probes_dev [nprobe][cell] (unit norm); grad_dev (optional) [num_params] */
+/* log K (without the constant -(1/n) log det J) and its parameter gradient
+ * for system A and nprobe probe vectors probes_dev [nprobe][cell] with
+ * E[y y^T] = I; grad_dev (optional) [num_params] */
 int stp_lpc_loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, const float* probes_dev,
                       int32_t nprobe, double* loss_host, float* grad_dev, void* stream);
 /* loss_grad followed by one Adam step (beta 0.9 / 0.999) with learning rate lr */

@@ -97,3 +97,13 @@ def ray_matrix(n, h, pts):
         cols.append(c[o]); vals.append(l[o])
         rp.append(rp[-1] + c.size)
     return np.asarray(rp, np.int64), np.concatenate(cols), np.concatenate(vals)
+
+
+def smooth_gradient(g, nx, ny, nz, sigma):
+    """SparseOps::smooth_gradient restated (direct 3-D sum)."""
+    g = np.ascontiguousarray(g, np.float64).reshape(-1)
+    out = np.zeros(nx * ny * nz)
+    if g.size < out.size:
+        return out
+    _lib().orc_smooth_gradient(C.c_int64(nx), C.c_int64(ny), C.c_int64(nz), C.c_double(sigma), _d(g), _d(out))
+    return out

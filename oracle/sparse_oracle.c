@@ -12,6 +12,7 @@
  *   SparseOps::conjugate_gradient   src/cpp/sparse/sparse_ops.cpp:8-50 (x0 = 0)
  *   SparseOps::jacobi_preconditioned_cg  sparse_ops.cpp:52-113 (x0 = 0)
  *   SparseOps::dot_product          sparse_ops.cpp:172-187 (4-way unrolled)
+ *   SparseOps::smooth_gradient      sparse_ops.cpp:115-158 (direct 3-D sum)
  * and adds (no reference counterpart):
  *   CGLS on (A^T A + lambda I) x = A^T b, plain or Jacobi preconditioned;
  *   Siddon / Amanatides-Woo exact straight-ray path lengths through a grid of
@@ -282,4 +283,33 @@ int64_t orc_siddon(const int64_t* n, double h, const double* p0, const double* p
                                                                           : ((double)knext[a] * h - p0[a]) / d[a];
     }
     return m;
+}
+
+/* SparseOps::smooth_gradient (sparse_ops.cpp:115-158): for every cell, the
+ * sum over the (2r+1)^3 cube (r = (int)(3 sigma)) of in-range cells of
+ * exp(-(dx^2+dy^2+dz^2) / (2 sigma^2)) * g, divided by the in-range weight
+ * sum; x-major, z contiguous. */
+void orc_smooth_gradient(int64_t nx, int64_t ny, int64_t nz, double sigma, const double* g, double* out) {
+    const int64_t r = (int64_t)(3 * sigma);
+    for (int64_t ix = 0; ix < nx; ++ix)
+        for (int64_t iy = 0; iy < ny; ++iy)
+            for (int64_t iz = 0; iz < nz; ++iz) {
+                double s = 0.0, ws = 0.0;
+                for (int64_t dx = -r; dx <= r; ++dx) {
+                    const int64_t jx = ix + dx;
+                    if (jx < 0 || jx >= nx) continue;
+                    for (int64_t dy = -r; dy <= r; ++dy) {
+                        const int64_t jy = iy + dy;
+                        if (jy < 0 || jy >= ny) continue;
+                        for (int64_t dz = -r; dz <= r; ++dz) {
+                            const int64_t jz = iz + dz;
+                            if (jz < 0 || jz >= nz) continue;
+                            const double w = exp(-(double)(dx * dx + dy * dy + dz * dz) / (2.0 * sigma * sigma));
+                            s += w * g[(jx * ny + jy) * nz + jz];
+                            ws += w;
+                        }
+                    }
+                }
+                out[(ix * ny + iy) * nz + iz] = s / ws;
+            }
 }

@@ -1,0 +1,308 @@
+// Test program for the C++ drop-in layer: calls the reference-named classes
+// (WaveEquationSolver, CSRMatrix, SparseOps, RTMEngine, FaultTolerance,
+// load_config, CartesianGrid) the way code written against the reference
+// does, reading inputs from and writing outputs to raw binary files in a
+// directory, so that tests/test_dropin_cpp*.py can compare them with the
+// oracle.
+//
+//   dropin_driver config <yaml>            print the parsed configuration
+//   dropin_driver grid                     print grid index checks
+//   dropin_driver fd <dir>                 forward / step / adjoint propagation
+//   dropin_driver sparse <dir>             assembly, SpMV, CG, Jacobi-PCG, CGLS
+//   dropin_driver rtm <dir>                imaging condition, finalize, run_rtm
+//   dropin_driver health <dir>             count / clamp / repair of a field
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "core/config.h"
+#include "core/fault_tolerance.h"
+#include "core/grid.h"
+#include "core/mpi_context.h"
+#include "rtm/rtm_engine.h"
+#include "solver/wave_equation.h"
+#include "sparse/csr_matrix.h"
+#include "sparse/sparse_ops.h"
+
+using namespace seismic;
+
+namespace {
+
+template <typename T>
+std::vector<T> read(const std::string& path) {
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
+    if (!f) throw std::runtime_error("cannot read " + path);
+    const std::streamsize n = f.tellg();
+    f.seekg(0);
+    std::vector<T> v(static_cast<size_t>(n) / sizeof(T));
+    f.read(reinterpret_cast<char*>(v.data()), n);
+    return v;
+}
+
+template <typename T>
+void write(const std::string& path, const T* p, size_t n) {
+    std::ofstream f(path, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(p), static_cast<std::streamsize>(n * sizeof(T)));
+}
+
+template <typename T>
+void write(const std::string& path, const std::vector<T>& v) { write(path, v.data(), v.size()); }
+
+// key = value lines
+double param(const std::string& dir, const std::string& key) {
+    std::ifstream f(dir + "/params.txt");
+    std::string k;
+    double v;
+    while (f >> k >> v)
+        if (k == key) return v;
+    throw std::runtime_error("missing parameter " + key);
+}
+
+const char* step_name(StepResult s) {
+    switch (s) {
+        case StepResult::OK: return "OK";
+        case StepResult::NAN_DETECTED_AND_REPAIRED: return "NAN_DETECTED_AND_REPAIRED";
+        case StepResult::CFL_VIOLATION: return "CFL_VIOLATION";
+        default: return "FATAL_DIVERGENCE";
+    }
+}
+
+void print_result(const char* tag, const PropagationResult& r) {
+    std::printf("%s status=%s steps=%d ok=%d\n", tag, step_name(r.status), r.timesteps_completed,
+                r.completed_successfully ? 1 : 0);
+}
+
+int cmd_config(const std::string& path) {
+    const TomographyConfig c = load_config(path);
+    std::printf("solver.frequency %.17g\nsolver.dt %.17g\nsolver.nt %d\nsolver.water_depth %.17g\n",
+                c.solver.frequency, c.solver.dt, c.solver.nt, c.solver.water_depth);
+    std::printf("solver.seafloor_gradient %.17g\nsolver.sponge_width %d\nsolver.sponge_damping %.17g\n",
+                c.solver.seafloor_gradient, c.solver.sponge_width, c.solver.sponge_damping);
+    std::printf("solver.snapshot_interval %d\nsolver.output_dir %s\nsolver.fd_order %d\n", c.solver.snapshot_interval,
+                c.solver.output_dir.c_str(), c.solver.fd_order);
+    std::printf("model.vp_min %.17g\nmodel.vp_max %.17g\nmodel.vs_min %.17g\nmodel.vs_max %.17g\n", c.model.vp_min,
+                c.model.vp_max, c.model.vs_min, c.model.vs_max);
+    std::printf("model.rho_min %.17g\nmodel.rho_max %.17g\nmodel.initial_vp %.17g\nmodel.initial_vs %.17g\n",
+                c.model.rho_min, c.model.rho_max, c.model.initial_vp, c.model.initial_vs);
+    std::printf("model.initial_rho %.17g\n", c.model.initial_rho);
+    std::printf("inversion.max_iterations %d\ninversion.step_length %.17g\n", c.inversion.max_iterations,
+                c.inversion.step_length);
+    std::printf("inversion.gradient_smoothing_sigma %.17g\ninversion.regularization_lambda %.17g\n",
+                c.inversion.gradient_smoothing_sigma, c.inversion.regularization_lambda);
+    std::printf("inversion.misfit_tolerance %.17g\ninversion.misfit_type %s\n", c.inversion.misfit_tolerance,
+                c.inversion.misfit_type.c_str());
+    std::printf("inversion.gradient_precondition %d\ninversion.line_search_max %d\n",
+                c.inversion.gradient_precondition, c.inversion.line_search_max);
+    std::printf("inversion.line_search_factor %.17g\n", c.inversion.line_search_factor);
+    std::printf("source_positions %zu\n", c.source_positions.size());
+    return 0;
+}
+
+int cmd_grid() {
+    GridParams p{10, 7, 5, 10.0, 20.0, 5.0, 1.0, 2.0, 3.0, 2};
+    const int coords[3] = {1, 0, 0}, dims[3] = {3, 1, 1};
+    CartesianGrid g(p, coords, dims);
+    const LocalGrid& l = g.local();
+    std::printf("nx_local %lld offset_x %lld nx_with_halo %lld total_with_halo %lld\n", (long long)l.nx_local,
+                (long long)l.offset_x, (long long)l.nx_with_halo, (long long)l.total_cells_with_halo);
+    std::printf("index_with_halo(2,3,4) %lld\n", (long long)g.local_index_with_halo(2, 3, 4));
+    std::printf("is_halo(0,0,0) %d is_halo(2,2,2) %d x_coord(1) %.17g\n", g.is_halo(0, 0, 0) ? 1 : 0,
+                g.is_halo(2, 2, 2) ? 1 : 0, g.x_coord(1));
+    MpiContext mpi;
+    std::printf("rank %d size %d neighbor0 %d allreduce %.1f\n", mpi.config().rank, mpi.config().size,
+                mpi.config().neighbors[0], mpi.allreduce_sum(2.5));
+    return 0;
+}
+
+int cmd_fd(const std::string& dir) {
+    const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
+    const int hw = (int)param(dir, "hw");
+    GridParams gp{nx, ny, nz, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0, hw};
+    CartesianGrid grid(gp);
+    MpiContext mpi;
+    SolverConfig cfg = load_config(dir + "/none.yaml").solver;
+    cfg.dt = 1e-3;
+    cfg.nt = (int)param(dir, "nt");
+    cfg.sponge_width = (int)param(dir, "sponge_width");
+    cfg.sponge_damping = 0.015;
+    cfg.snapshot_interval = (int)param(dir, "snapshot_interval");
+    cfg.fd_order = (int)param(dir, "order");
+    const std::vector<double> vel = read<double>(dir + "/vel.f64");
+    const std::vector<double> w = read<double>(dir + "/wavelet.f64");
+    const std::vector<int64_t> rcv = read<int64_t>(dir + "/rcv.i64");
+
+    WaveEquationSolver s(mpi, grid, cfg);
+    s.set_velocity_field(vel.data(), (int64_t)vel.size());
+    s.set_receivers(rcv);
+    std::printf("order %d cfl %d\n", s.order(), s.verify_cfl() ? 1 : 0);
+    const PropagationResult r = s.forward_propagate(w);
+    print_result("forward", r);
+    write(dir + "/cur.f64", s.field().current);
+    write(dir + "/prev.f64", s.field().previous);
+    write(dir + "/traces.f64", s.receiver_traces());
+    write(dir + "/snapshots.f64", s.field().snapshots);
+    std::printf("snapshots %lld energy %.17g\n", (long long)s.snapshots_captured(), s.compute_energy());
+
+    // the same run by hand: inject at the centre, then step (wave_equation.cpp:118-125)
+    WaveEquationSolver m(mpi, grid, cfg);
+    m.set_velocity_field(vel.data(), (int64_t)vel.size());
+    for (int t = 0; t < cfg.nt; ++t) {
+        if (t < (int)w.size()) m.inject_source(nx / 2, ny / 2, nz / 2, {w[(size_t)t]});
+        m.step_forward(t);
+    }
+    write(dir + "/cur_steps.f64", m.field().current);
+    write(dir + "/prev_steps.f64", m.field().previous);
+
+    // adjoint_propagate continues from the forward state (no reset)
+    const PropagationResult ra = s.adjoint_propagate(w);
+    print_result("adjoint", ra);
+    write(dir + "/cur_adj.f64", s.field().current);
+
+    // CFL violation: 10x the velocity fails verify_cfl before any step
+    std::vector<double> fast(vel);
+    for (double& v : fast) v *= 10.0;
+    WaveEquationSolver f(mpi, grid, cfg);
+    f.set_velocity_field(fast.data(), (int64_t)fast.size());
+    print_result("fast", f.forward_propagate(w));
+    return 0;
+}
+
+int cmd_sparse(const std::string& dir) {
+    const int64_t rows = (int64_t)param(dir, "rows"), cols = (int64_t)param(dir, "cols");
+    const std::vector<int64_t> r = read<int64_t>(dir + "/r.i64"), c = read<int64_t>(dir + "/c.i64");
+    const std::vector<double> v = read<double>(dir + "/v.f64");
+    std::vector<SparseEntry> e(r.size());
+    for (size_t k = 0; k < r.size(); ++k) e[k] = SparseEntry{r[k], c[k], v[k]};
+    CSRMatrix A;
+    A.build_from_triplets(rows, cols, e);
+    write(dir + "/rp.i64", A.row_ptr().data(), A.row_ptr().size());
+    write(dir + "/ci.i64", A.col_ind().data(), A.col_ind().size());
+    write(dir + "/val.f64", A.values().data(), A.values().size());
+    std::printf("nnz %lld frobenius %.17g\n", (long long)A.nnz(), A.frobenius_norm());
+
+    const std::vector<double> xv = read<double>(dir + "/x.f64");
+    AlignedVector<double> x(xv.size());
+    for (size_t i = 0; i < xv.size(); ++i) x[i] = xv[i];
+    const AlignedVector<double> y = A.spmv(x);
+    write(dir + "/y.f64", y.data(), y.size());
+    const std::vector<double> uv = read<double>(dir + "/u.f64");
+    AlignedVector<double> u(uv.size());
+    for (size_t i = 0; i < uv.size(); ++i) u[i] = uv[i];
+    const AlignedVector<double> yt = A.spmv_transpose(u);
+    write(dir + "/yt.f64", yt.data(), yt.size());
+    std::printf("dot %.17g\n", SparseOps::dot_product(x, x));
+
+    // square SPD system: the 7-point FD matrix shifted to be diagonally dominant
+    const std::vector<double> vel = read<double>(dir + "/fdvel.f64");
+    const int64_t n = (int64_t)param(dir, "fdn");
+    CSRMatrix F;
+    F.build_fd_stencil_7point(n, n, n, vel.data(), 10.0, 10.0, 10.0, 1e-3);
+    write(dir + "/fd_rp.i64", F.row_ptr().data(), F.row_ptr().size());
+    write(dir + "/fd_ci.i64", F.col_ind().data(), F.col_ind().size());
+    write(dir + "/fd_val.f64", F.values().data(), F.values().size());
+    F.scale(-1.0);
+    CSRMatrix H = SparseOps::build_hessian_approx(F, 6.0);   // 6 I - (FD matrix): SPD
+    AlignedVector<double> b(static_cast<size_t>(H.rows()), 1.0);
+    SolveInfo i1, i2;
+    const AlignedVector<double> x1 = SparseOps::conjugate_gradient(H, b, 1000, 1e-10, &i1);
+    const AlignedVector<double> x2 = SparseOps::jacobi_preconditioned_cg(H, b, 1000, 1e-10, &i2);
+    write(dir + "/cg.f64", x1.data(), x1.size());
+    write(dir + "/pcg.f64", x2.data(), x2.size());
+    write(dir + "/h_val.f64", H.values().data(), H.values().size());
+    std::printf("cg iters %d converged %d pcg iters %d converged %d\n", i1.iterations, i1.converged ? 1 : 0,
+                i2.iterations, i2.converged ? 1 : 0);
+
+    // CGLS on the rectangular triplet matrix
+    SolveInfo i3;
+    const AlignedVector<double> x3 = SparseOps::cgls(A, u, 1e-2, 1e-8, 5000, CglsPreconditioner::JACOBI, &i3);
+    write(dir + "/cgls.f64", x3.data(), x3.size());
+    std::printf("cgls iters %d converged %d true_rel %.3e\n", i3.iterations, i3.converged ? 1 : 0,
+                i3.true_rel_residual);
+    return 0;
+}
+
+int cmd_rtm(const std::string& dir) {
+    const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
+    const int hw = (int)param(dir, "hw");
+    const std::vector<double> f = read<double>(dir + "/f.f64"), a = read<double>(dir + "/a.f64");
+    RTMEngine eng(nx, ny, nz, 10.0, 10.0, 10.0, hw);
+    eng.apply_image_condition(f.data(), a.data(), ny + 2 * hw, nz + 2 * hw, 0);
+    eng.apply_image_condition(a.data(), f.data(), ny + 2 * hw, nz + 2 * hw, 1);
+    eng.finalize_image();
+    const RTMImageCondition& res = eng.result();
+    write(dir + "/image.f64", res.image);
+    write(dir + "/illum.f64", res.illumination);
+    write(dir + "/refl.f64", res.reflectivity);
+
+    // run_rtm through the solver: source wavelet and receiver residuals
+    GridParams gp{nx, ny, nz, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0, hw};
+    CartesianGrid grid(gp);
+    MpiContext mpi;
+    SolverConfig cfg = load_config(dir + "/none.yaml").solver;
+    cfg.dt = 1e-3;
+    cfg.nt = (int)param(dir, "nt");
+    cfg.sponge_width = 6;
+    cfg.fd_order = 8;
+    const std::vector<double> vel = read<double>(dir + "/vel.f64");
+    const std::vector<double> w = read<double>(dir + "/wavelet.f64");
+    const std::vector<int64_t> rcv = read<int64_t>(dir + "/rcv.i64");
+    const std::vector<double> resid = read<double>(dir + "/resid.f64");
+    WaveEquationSolver s(mpi, grid, cfg);
+    s.set_velocity_field(vel.data(), (int64_t)vel.size());
+    s.set_receivers(rcv);
+    print_result("rtm_disabled", s.run_rtm(w, resid));
+    RTMConfig rc;
+    rc.enable_rtm = true;
+    s.configure_rtm(rc);
+    print_result("rtm", s.run_rtm(w, resid));
+    write(dir + "/rtm_image.f64", s.rtm_engine()->result().image);
+    write(dir + "/rtm_illum.f64", s.rtm_engine()->result().illumination);
+    write(dir + "/rtm_traces.f64", s.receiver_traces());
+    return 0;
+}
+
+int cmd_health(const std::string& dir) {
+    std::vector<double> d = read<double>(dir + "/field.f64");
+    const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
+    std::printf("nonfinite %lld\n", (long long)FaultTolerance::count_nan_inf(d.data(), (int64_t)d.size()));
+    std::vector<double> c(d);
+    FaultTolerance::clamp_field(c.data(), (int64_t)c.size(), -1.0, 1.0);
+    write(dir + "/clamped.f64", c);
+    FaultTolerance ft;
+    std::vector<double> rep(d);
+    ft.repair_field_nan(rep.data(), (int64_t)rep.size(), nx, ny, nz, 1, -7.0);
+    write(dir + "/repaired.f64", rep);
+    std::vector<double> cur(d), prev(d);
+    const bool ok = ft.attempt_recovery(cur.data(), prev.data(), (int64_t)cur.size(), nx, ny, nz, 1, 3);
+    std::printf("recovered %d cells_repaired %lld log %zu\n", ok ? 1 : 0, (long long)ft.stats().cells_repaired,
+                ft.fault_log().size());
+    write(dir + "/recovered.f64", cur);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: dropin_driver config|grid|fd|sparse|rtm|health [arg]\n");
+        return 2;
+    }
+    const std::string cmd = argv[1], arg = argc > 2 ? argv[2] : "";
+    try {
+        if (cmd == "config") return cmd_config(arg);
+        if (cmd == "grid") return cmd_grid();
+        if (cmd == "fd") return cmd_fd(arg);
+        if (cmd == "sparse") return cmd_sparse(arg);
+        if (cmd == "rtm") return cmd_rtm(arg);
+        if (cmd == "health") return cmd_health(arg);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
+    return 2;
+}

@@ -195,3 +195,29 @@ def test_siddon_matches_fine_sampling():
         got = np.zeros(np.prod(n))
         np.add.at(got, cells, lens)
         np.testing.assert_allclose(got, est, atol=3 * L / m + 1e-12)
+
+
+def test_smooth_gradient_closed_forms():
+    """Constant fields are fixed points (renormalised weights); an interior
+    delta returns the normalised truncated Gaussian, whose centre value is
+    1 / (sum_{|d| <= r} exp(-d^2 / (2 sigma^2)))^3 with r = int(3 sigma);
+    the filter is linear and symmetric (self-adjoint away from the edges)."""
+    n = (13, 14, 13)
+    V = int(np.prod(n))
+    for sigma in (0.4, 1.0, 1.2):
+        assert np.allclose(osp.smooth_gradient(np.full(V, 2.5), *n, sigma), 2.5, rtol=1e-14)
+        r = int(3 * sigma)
+        z1 = sum(np.exp(-d * d / (2 * sigma ** 2)) for d in range(-r, r + 1))
+        d = np.zeros(n)
+        d[6, 7, 6] = 1.0    # every cell within r of it has its whole cube inside the grid
+        out = osp.smooth_gradient(d.ravel(), *n, sigma).reshape(n)
+        assert out[6, 7, 6] == pytest.approx(1.0 / z1 ** 3, rel=1e-12)
+        assert out[6, 7, 7] == pytest.approx(np.exp(-1 / (2 * sigma ** 2)) / z1 ** 3, rel=1e-12)
+        assert out[7, 8, 7] == pytest.approx(np.exp(-3 / (2 * sigma ** 2)) / z1 ** 3, rel=1e-12)
+        assert out.sum() == pytest.approx(1.0, rel=1e-12)
+        assert np.count_nonzero(out) == (2 * r + 1) ** 3
+    # sigma < 1/3: radius 0, identity
+    x = np.random.default_rng(0).standard_normal(V)
+    assert np.array_equal(osp.smooth_gradient(x, *n, 0.3), x)
+    # short input -> zeros (sparse_ops.cpp:122)
+    assert not osp.smooth_gradient(np.ones(V - 1), *n, 1.0).any()
