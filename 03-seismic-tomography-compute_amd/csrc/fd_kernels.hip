@@ -509,9 +509,13 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
     X(8, 128, 8, 1, 128, true)        \
     X(9, 256, 8, 1, 64, false)        \
     X(10, 128, 12, 1, 96, false)      \
-    X(11, 64, 16, 2, 64, false)
+    X(11, 64, 16, 2, 64, false)       \
+    X(12, 128, 32, 2, 64, false)      \
+    X(13, 128, 32, 4, 64, false)      \
+    X(14, 64, 32, 1, 64, false)       \
+    X(15, 128, 24, 2, 64, false)
 
-int fd_num_variants() { return 12; }
+int fd_num_variants() { return 16; }
 
 // Fixed tiles of the other paths (must match fd_launch_step below).
 #define STP_FD_FIXED(X)                 \

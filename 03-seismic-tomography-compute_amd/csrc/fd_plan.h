@@ -18,8 +18,14 @@ struct stp_fd_plan {
     int* d_rcv_id = nullptr;
     // sources
     int64_t* d_src = nullptr;
+    int64_t* d_rcv_plain = nullptr;   // receiver r -> cell (two-step path)
     // scratch
     unsigned int* d_word = nullptr;
+    // two-step path: tiling and the two output fields of a pair launch
+    bool pair_ok = false;
+    int variant2 = 0;                 // tile of the two-step kernel (STP_FD2_TILE)
+    int ntz2 = 0, nty2 = 0, nxc2 = 0;
+    float* d_pair = nullptr;          // 2 * nshots * ncell floats, allocated on first use
 };
 
 namespace stp {

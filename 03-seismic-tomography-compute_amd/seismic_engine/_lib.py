@@ -107,6 +107,10 @@ def lib() -> C.CDLL:
     """Load the library once; raise if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime with the same soname as ROCm's; load
+        # it first so that this library binds to the same runtime (one set of
+        # streams and device allocations in the process)
+        import torch  # noqa: F401
         if not LIB_PATH.exists():
             raise SeismicError(
                 f"{LIB_PATH} is missing: build it with `python __graft_entry__.py build` "

@@ -77,9 +77,18 @@ def clamp_field_safe(data, lo: float = -1e10, hi: float = 1e10):
     """Clamp to [lo, hi], non-finite values to 0 (FaultTolerance::clamp_field)."""
     d, host = _device_f64(data)
     call("stp_clamp_field_f64", ptr(d), C.c_int64(d.numel()), C.c_double(lo), C.c_double(hi), stream_handle())
+    return _write_back(data, d, host)
+
+
+def _write_back(data, d, host):
+    """Copy the device result into the host array; like the reference's
+    forcecast binding, a float64 C-contiguous input is updated in place and
+    returned itself."""
     if host is None:
         return data
     host[:] = d.cpu().numpy()
+    if isinstance(data, np.ndarray) and np.shares_memory(host, data):
+        return data
     return host
 
 
@@ -91,7 +100,4 @@ def repair_field_nan(data, nx: int, ny: int, nz: int, halo_width: int = 1, repla
     d, host = _device_f64(data)
     call("stp_repair_field_nan_f64", ptr(d), C.c_int64(d.numel()), C.c_int64(nx), C.c_int64(ny), C.c_int64(nz),
          int(halo_width), C.c_double(replace_value), stream_handle())
-    if host is None:
-        return data
-    host[:] = d.cpu().numpy()
-    return host
+    return _write_back(data, d, host)

@@ -42,23 +42,47 @@ def pmc_traffic(n: int) -> tuple[float | None, str | None]:
     return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
-def cpu_baseline(threads: int, steps: int = 500):
+def cpu_baseline(threads: int, vel_h, n: int, steps: int):
     """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
-    on the 128^3 two-layer case for `steps` of its 500 steps."""
+    on the benchmark workload itself: the same n^3 model, order 8, sponge,
+    centre Ricker source and receivers, for the first `steps` time steps."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import numpy as np
     from oracle import fd as ofd
     from seismic_engine import synthetic
+    c = ofd.cfg((n, n, n), hw=4, order=8, sponge_width=20, sponge_damping=0.015)
+    w = synthetic.ricker(15.0, 1e-3, steps)
+    rcv = synthetic.line_receivers(n, n, n, 4, iy=n // 2, step=max(1, n // 256))
+    t0 = time.perf_counter()
+    _, _, _, res = ofd.forward(c, vel_h, w, steps, rcv=rcv)
+    dt = time.perf_counter() - t0
+    assert res.completed_successfully == 1
+    return float(n) ** 3 * steps / dt / 1e9, dt
+
+
+def accuracy_check(dev, threads: int, steps: int = 500):
+    """FP32 GPU propagation against the FP64 oracle on the 128^3 two-layer
+    case (full 500 steps): relative L2 error of the final field and traces."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import numpy as np
+    import torch
+    from oracle import fd as ofd
+    from seismic_engine import FDPropagator, synthetic
     n = (128, 128, 128)
     c = ofd.cfg(n, hw=4, order=8, sponge_width=20, sponge_damping=0.015)
     v = synthetic.pad_halo(synthetic.two_layer_velocity(*n, split_iz=64), 4)
     w = synthetic.ricker(15.0, 1e-3, steps)
     rcv = synthetic.surface_receivers(*n, 4)
-    t0 = time.perf_counter()
-    cur, _, tr, res = ofd.forward(c, v, w, steps, rcv=rcv)
-    dt = time.perf_counter() - t0
-    assert res.completed_successfully == 1
-    return 128**3 * steps / dt / 1e9, dt, cur, tr, v, w, rcv
+    cur_o, _, tr_o, res = ofd.forward(c, v, w, steps, rcv=rcv)
+    g = FDPropagator(*n, order=8, sponge_width=20, sponge_damping=0.015, device=dev)
+    g.set_velocity(v)
+    g.set_sources([g.index(64, 64, 64)])
+    g.set_receivers(rcv)
+    trg = g.propagate(steps, w)
+    torch.cuda.synchronize()
+    e = float(np.linalg.norm(g.cur[0].cpu().numpy() - cur_o) / np.linalg.norm(cur_o))
+    et = float(np.linalg.norm(trg[0].cpu().numpy() - tr_o) / np.linalg.norm(tr_o))
+    return {"case": f"128^3 two-layer, {steps} steps", "field_rel_l2_vs_fp64": e, "traces_rel_l2_vs_fp64": et}
 
 
 def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: float = 1e-6, reps: int = 3):
@@ -90,19 +114,36 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
     spmv_bytes = 8 * A.nnz + 4 * (A.rows + 1) + 4 * A.rows + 4 * A.cols
     out["spmv"] = {"us": round(us, 2), "gb_s": round(spmv_bytes / (us * 1e-6) / 1e9, 1),
                    "frac_hbm": round(spmv_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
-    for kind in ("jacobi", "none"):
+    from seismic_engine.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, train_recipe
+    if DEFAULT_WEIGHTS.exists():
+        net = LearnedPreconditioner.load(DEFAULT_WEIGHTS)
+        out["learned_weights"] = str(DEFAULT_WEIGHTS.relative_to(ROOT))
+    else:
+        log("no trained weights found: training the learned preconditioner now")
+        t0 = time.perf_counter()
+        net, tinfo = train_recipe()
+        out["learned_weights"] = f"trained in this run ({time.perf_counter() - t0:.1f} s)"
+    grid = (n, n, n)
+    for kind in ("jacobi", "none", "learned"):
         best = None
         for _ in range(reps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            x, info, _ = A.cgls(b, lam, tol, 20000, kind)
+            if kind == "learned":
+                net.prepare(A, grid, lam)     # coupling features + network: part of the solve time
+                torch.cuda.synchronize()
+                tprep = time.perf_counter() - t0
+            x, info, _ = A.cgls(b, lam, tol, 20000, kind, network=net if kind == "learned" else None)
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             if best is None or dt < best[0]:
-                best = (dt, info)
-        dt, info = best
+                best = (dt, info, tprep if kind == "learned" else 0.0)
+        dt, info, tp = best
         out[kind] = {"seconds": round(dt, 5), "iterations": info.iterations, "converged": bool(info.converged),
                      "true_rel_residual": info.true_rel_residual, "replacements": info.replacements}
+        if kind == "learned":
+            out[kind]["prepare_seconds"] = round(tp, 5)
+    out["learned_over_jacobi_time"] = round(out["learned"]["seconds"] / out["jacobi"]["seconds"], 4)
     return out
 
 
@@ -114,6 +155,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cg", action="store_true", help="skip the tomography-solve timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for PMC byte counters)")
@@ -213,26 +255,20 @@ def main() -> int:
         result["cg"] = cg_workload(dev)
 
     if not args.no_cpu_baseline:
+        vel_h = fd.velocity.double().cpu().numpy()
         del fd
         torch.cuda.empty_cache()
-        steps = 500
-        log(f"cpu baseline: FP64 oracle 128^3, {steps} steps, {args.cpu_threads} threads")
-        gcs, sec, cur_o, tr_o, v_h, w_np, rcv1 = cpu_baseline(args.cpu_threads, steps)
+        steps = args.cpu_steps
+        log(f"cpu baseline: FP64 oracle on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
+        gcs, sec = cpu_baseline(args.cpu_threads, vel_h, n, steps)
+        del vel_h
         result["cpu_baseline"] = {
             "value": round(gcs, 5), "unit": "Gcell-updates/s", "cores": args.cpu_threads, "kind": "port",
-            "sample": f"FP64 oracle (reference 4-pass step restated, OpenMP over x), 128^3 two-layer, "
-                      f"order 8, first {steps} of 500 steps, {sec:.1f} s"}
-        # FP32 GPU vs FP64 oracle on that same sample
-        g = FDPropagator(128, 128, 128, order=8, sponge_width=20, sponge_damping=0.015, device=dev)
-        g.set_velocity(v_h)
-        g.set_sources([g.index(64, 64, 64)])
-        g.set_receivers(rcv1)
-        trg = g.propagate(steps, w_np)
-        torch.cuda.synchronize()
-        e = float(np.linalg.norm(g.cur[0].cpu().numpy() - cur_o) / np.linalg.norm(cur_o))
-        et = float(np.linalg.norm(trg[0].cpu().numpy() - tr_o) / np.linalg.norm(tr_o))
-        result["accuracy"] = {"case": f"128^3 two-layer, {steps} steps", "field_rel_l2_vs_fp64": e,
-                              "traces_rel_l2_vs_fp64": et}
+            "sample": f"FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x) on the same "
+                      f"{n}^3 smooth random model, order 8, sponge 20, {len(rcv)} receivers: first {steps} of "
+                      f"{nt} steps, {sec:.1f} s"}
+        log("accuracy: 128^3 two-layer, 500 steps, FP32 GPU vs FP64 oracle")
+        result["accuracy"] = accuracy_check(dev, args.cpu_threads)
     print(json.dumps(result), flush=True)
     return 0
 

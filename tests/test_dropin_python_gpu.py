@@ -25,8 +25,10 @@ def test_health_functions_follow_inversion_call_pattern(cuda):
     out = repair_field_nan(repaired, 6, 5, 4, halo_width=1, replace_value=1500.0)
     ref = ofd.repair_field_nan(vp.ravel().copy(), 6, 5, 4, 1, 1500.0)
     assert np.array_equal(repaired.ravel(), ref) and np.array_equal(np.asarray(out).ravel(), ref)
-    c = clamp_field_safe(np.ascontiguousarray(vp.ravel()), lo=1600.0, hi=2900.0)
-    assert np.array_equal(c, np.where(np.isfinite(vp.ravel()), np.clip(vp.ravel(), 1600.0, 2900.0), 0.0))
+    flat = vp.ravel().copy()
+    c = clamp_field_safe(flat, lo=1600.0, hi=2900.0)     # in place, like the forcecast binding
+    want = np.where(np.isfinite(vp.ravel()), np.clip(vp.ravel(), 1600.0, 2900.0), 0.0)
+    assert c is flat and np.array_equal(flat, want)
     # default limits +-1e10 and torch tensors in place
     t = torch.tensor([1e11, -1e12, float("nan"), 3.0], dtype=torch.float64, device=cuda)
     clamp_field_safe(t)

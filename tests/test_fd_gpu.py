@@ -169,3 +169,80 @@ def test_invalid_arguments_raise(cuda):
         fd.set_sources([fd.index(1, 1, 1), fd.index(2, 2, 2)])   # one per shot
     with pytest.raises(SeismicError):
         FDPropagator(16, 16, 16, order=4, device=cuda)
+
+
+def _run_fd(cuda, n, nt, chunks, single, nshots=1, seed=5):
+    import os
+    import torch
+    old = os.environ.pop("STP_FD_PAIR", None)
+    if not single:
+        os.environ["STP_FD_PAIR"] = "1"
+    try:
+        fd = FDPropagator(*n, order=8, sponge_width=6, nshots=nshots, device=cuda)
+    finally:
+        os.environ.pop("STP_FD_PAIR", None)
+        if old is not None:
+            os.environ["STP_FD_PAIR"] = old
+    vel = synthetic.smooth_random_velocity(*n, seed=seed, sigma=3.0, vmin=1500.0, vmax=3500.0)
+    fd.set_velocity(vel)
+    srcs = [fd.index(n[0] // 2 + s, n[1] // 3, n[2] // 2 - s) for s in range(nshots)]
+    fd.set_sources(srcs)
+    rcv = synthetic.line_receivers(*n, 4, iy=n[1] // 2, step=3)
+    rcv = np.concatenate([rcv, [srcs[0]]])          # one receiver on a source cell
+    fd.set_receivers(rcv)
+    w = synthetic.ricker(20.0, 1e-3, nt)
+    trs = [fd.propagate(k, w) for k in chunks]
+    return fd.cur.clone(), fd.prev.clone(), torch.cat(trs, dim=2)
+
+
+@pytest.mark.parametrize("n,chunks,nshots", [
+    ((70, 45, 68), (40,), 1),          # partial tiles in y and z, even pair count
+    ((70, 45, 68), (6, 11, 23), 1),    # odd pair counts and an odd step count
+    ((36, 30, 28), (18,), 2),          # two shots
+    ((132, 33, 72), (10,), 1),         # two x chunks of the pair kernel
+])
+def test_two_step_kernel_equals_single_steps(n, chunks, nshots, cuda):
+    """The pair kernel (temporal blocking) gives bit-identical fields and
+    traces to the single-step kernel, whatever the split of the steps."""
+    import torch
+    nt = sum(chunks)
+    c1, p1, t1 = _run_fd(cuda, n, nt, chunks, single=True, nshots=nshots)
+    c2, p2, t2 = _run_fd(cuda, n, nt, chunks, single=False, nshots=nshots)
+    assert torch.equal(c1, c2)
+    assert torch.equal(t1, t2)
+    assert torch.equal(p1, p2)
+
+
+def test_two_step_kernel_guards_and_status(cuda):
+    """Non-finite values inside a pair: same guards and status words as the
+    oracle's step-by-step run (4 steps = two pair launches)."""
+    import os
+    import torch
+    n = (20, 20, 20)
+    hw = 4
+    c = ofd.cfg(n, hw=hw, order=8, sponge_width=0)
+    nh = ofd.nh(c)
+    rng = np.random.default_rng(12)
+    u = np.zeros(nh)
+    u[hw:-hw, hw:-hw, hw:-hw] = rng.standard_normal(n)
+    p = u * 0.9
+    u[10, 10, 10] = np.nan
+    p[14, 12, 9] = np.inf
+    v_h = np.full(nh, 2000.0)
+    cur_o, prev_o = u.copy(), p.copy()
+    s_o, _ = ofd.run(c, v_h, cur_o, prev_o, 4)
+    os.environ["STP_FD_PAIR"] = "1"
+    try:
+        fd = FDPropagator(*n, order=8, sponge_width=0, device=cuda)
+    finally:
+        os.environ.pop("STP_FD_PAIR", None)
+    fd.set_velocity(v_h)
+    fd.cur[0] = torch.as_tensor(u, dtype=torch.float32)
+    fd.prev[0] = torch.as_tensor(p, dtype=torch.float32)
+    _, st = fd.propagate(4, status=True)
+    st = st.cpu().numpy()
+    assert s_o == 1
+    assert st[0] == 1 and st[1] == 1
+    got = fd.cur[0].cpu().numpy()
+    assert np.isfinite(got).all()
+    assert rel_l2(got, cur_o) < TOL
