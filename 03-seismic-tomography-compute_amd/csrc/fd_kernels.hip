@@ -494,9 +494,10 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
 }
 
 // Tile shapes (TZ, TY, TYR, LX, non-temporal P/C/N) compiled for the float4
-// path.  Variant 6 (128 x 16 cells, one row per thread, 512 threads) is the
-// default (fastest measured at 512^3); the others are kept for tuning
-// through STP_FD_TILE=<index>.
+// path.  Variant 20 (128 x 16 cells, one row per thread, 512 threads, x chunks
+// of 256 planes) is the default: fastest measured at 512^3 (398.8 us/step
+// against 450.6 for 64-plane chunks, since each chunk re-reads 2R halo planes
+// of u^t).  The others are kept for tuning through STP_FD_TILE=<index>.
 #define STP_FD_VARIANTS(X)            \
     X(0, 128, 8, 1, 64, false)        \
     X(1, 64, 16, 1, 64, false)        \
@@ -513,9 +514,23 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
     X(12, 128, 32, 2, 64, false)      \
     X(13, 128, 32, 4, 64, false)      \
     X(14, 64, 32, 1, 64, false)       \
-    X(15, 128, 24, 2, 64, false)
+    X(15, 128, 24, 2, 64, false)      \
+    X(16, 128, 32, 2, 128, false)     \
+    X(17, 128, 32, 2, 64, true)       \
+    X(18, 128, 16, 1, 128, false)     \
+    X(19, 128, 32, 2, 256, false)     \
+    X(20, 128, 16, 1, 256, false)     \
+    X(21, 64, 32, 2, 256, false)      \
+    X(22, 128, 32, 4, 128, false)     \
+    X(23, 256, 16, 2, 128, false)     \
+    X(24, 128, 16, 1, 256, true)      \
+    X(25, 128, 8, 1, 512, false)      \
+    X(26, 64, 16, 1, 512, false)      \
+    X(27, 128, 16, 1, 512, false)     \
+    X(28, 256, 16, 2, 256, false)     \
+    X(29, 256, 8, 1, 256, false)
 
-int fd_num_variants() { return 16; }
+int fd_num_variants() { return 30; }
 
 // Fixed tiles of the other paths (must match fd_launch_step below).
 #define STP_FD_FIXED(X)                 \

@@ -7,7 +7,7 @@
 
 struct stp_fd_plan {
     stp_fd_config cfg;
-    int R = 0, VW = 1, variant = 6, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
+    int R = 0, VW = 1, variant = 20, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
     int nxh = 0, nyh = 0, nzh = 0;
     int64_t ncell = 0;
     float cx[5] = {0}, cy[5] = {0}, cz[5] = {0};

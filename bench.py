@@ -246,7 +246,7 @@ def main() -> int:
                    "step": "one full forward propagation"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fd_step_kernel<4,4,128,16,1,64>", "bytes_per_cell_update": BYTES_PER_CELL,
+                     "kernel": "fd_step_kernel<4,4,128,16,1,256>", "bytes_per_cell_update": BYTES_PER_CELL,
                      "avg_launch_us": round(launch_s * 1e6, 2), "traffic_source": traffic_src},
     }
 
