@@ -528,9 +528,12 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
     X(26, 64, 16, 1, 512, false)      \
     X(27, 128, 16, 1, 512, false)     \
     X(28, 256, 16, 2, 256, false)     \
-    X(29, 256, 8, 1, 256, false)
+    X(29, 256, 8, 1, 256, false)      \
+    X(30, 128, 32, 1, 128, false)     \
+    X(31, 64, 64, 2, 128, false)      \
+    X(32, 128, 24, 1, 192, false)
 
-int fd_num_variants() { return 30; }
+int fd_num_variants() { return 33; }
 
 // Fixed tiles of the other paths (must match fd_launch_step below).
 #define STP_FD_FIXED(X)                 \

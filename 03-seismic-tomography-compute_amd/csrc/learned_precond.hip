@@ -5,30 +5,32 @@
 // Jacobi-scaled couplings of N to its 26 neighbours,
 //   J[i][i+d] = N[i][i+d] / sqrt(N[i][i] N[i+d][i+d]),
 // plus a constant channel (which marks the model boundary through the zero
-// padding), and it outputs the 63 coefficients of row i of an upper-triangular
-// factor G (taps d in [-2,2]^3 lexicographically >= 0, the centre first; the
-// diagonal is exp(output) > 0).  The preconditioner is
-//   M = D^-1/2 G G^T D^-1/2,   D = diag(N),
-// symmetric positive definite for any weights, so the CG theory holds; the
-// network runs once per system and each iteration costs two 63-tap stencils.
+// padding), and it outputs, per cell, a log scale o0 and the log weights
+// o1..o3 of the three grid edges (i, i+x), (i, i+y), (i, i+z).  With
+//   s_i = exp(o0_i) / sqrt(N_ii),   w_e = W0 exp(o_e)  (edges inside the grid),
+// the preconditioner is
+//   M = S (I + L_w) S,   S = diag(s),  L_w = sum_e w_e (e_i - e_j)(e_i - e_j)^T,
+// a weighted graph Laplacian on the 7-point grid graph: symmetric positive
+// definite for any weights (I + L_w >= I), so the CG theory holds, and one
+// application is a single 7-point stencil.  Untrained (zero output layer) it
+// is D^-1/2 (I + W0 L) D^-1/2 with L the grid Laplacian: Jacobi scaling
+// followed by a high-pass that lifts the poorly constrained short-wavelength
+// components (the normal operator of straight rays damps wavenumber k like
+// 1/|k|) towards the rest of the spectrum.
 //
-// Training minimises Kaporin's condition number of G^T J G
-// (J = D^-1/2 N D^-1/2):
-//   K = (tr(G^T J G) / n) / det(G^T J G)^(1/n),
-//   log K = log(tr(G^T J G) / n) - (2/n) sum_i log G_ii - (1/n) log det J,
-// with the trace estimated from Rademacher probe vectors y, E[y y^T] = I:
-//   tr(G^T J G) ~ (1/P) sum_y (G y)^T J (G y).
-// K >= 1 with equality iff G G^T is a multiple of J^-1, it bounds the CG
-// iteration count (Kaporin 1994), and it is invariant to the scale of G.  Its
-// minimiser over a fixed pattern is the factorised sparse approximate inverse
-// (FSAI) of J; the network predicts such a factor from local couplings.
+// Training differentiates the CG iteration itself: the library gives the
+// gradient of <g, M r> with respect to the network output for any pair of
+// vectors (stp_lpc_backward_accumulate), the host side unrolls preconditioned
+// CG on training systems and back-propagates the energy-norm errors of its
+// iterates, and stp_lpc_backward / stp_lpc_adam_step take that output
+// gradient through the network.
 //
 // Network: 3 levels, 16/32/64 channels, 3x3x3 convolutions, ReLU, average
 // pooling down, nearest upsampling and skip concatenation up (conv3d.hip),
 // BF16 weights and activations, FP32 accumulation and FP32 master weights
 // (Adam).  Layers, [cout][27][cin]:
 //   e1 32->16, e2 16->32 (after pool), e3 32->64 (after pool),
-//   d2 (64+32)->32, d1 (32+16)->16, o 16->64 (63 used, no ReLU).
+//   d2 (64+32)->32, d1 (32+16)->16, o 16->16 (4 used, no ReLU).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,10 +53,10 @@ namespace {
 
 constexpr int NFEAT = 32;   // input channels (26 couplings + constant + padding)
 constexpr int NTAP = 27;    // 3x3x3 couplings and convolution taps
-constexpr int RG = 2;       // stencil radius of the factor G
-constexpr int NG = 63;      // taps of G: offsets in [-2,2]^3 lexicographically >= 0
-constexpr int NOUT = 64;    // network output channels (NG used)
-constexpr int PARTS = 1024; // FP64 block partial sums per reduction
+constexpr int NOUT = 16;    // network output channels (NCOEF used)
+constexpr int NCOEF = 4;    // per cell: scale s and the weights of the +x, +y, +z edges
+constexpr float W0 = 1.0f;  // edge weight of an untrained network
+constexpr float OCLAMP = 30.f;  // outputs are clamped to +-OCLAMP before the exponential
 
 struct LayerDef {
     int cin, cout;
@@ -143,141 +145,134 @@ __global__ void features_kernel(int n0, int n1, int n2, const double* nloc, doub
     }
 }
 
-// ------------------------------------------------------------------ stencils of G
-// Tap k of G couples cell i to cell i + d_k, d_k = the (62 + k)-th offset of
-// [-2,2]^3 in lexicographic (x, y, z) order; k = 0 is the diagonal.
-__device__ __forceinline__ void g_offset(int k, int& dx, int& dy, int& dz) {
-    const int l = 62 + k;
-    dx = l / 25 - RG;
-    dy = (l / 5) % 5 - RG;
-    dz = l % 5 - RG;
+// ------------------------------------------------------------------ M = S (I + L_w) S
+// network output (SoA, NOUT rows of V) -> coef[4][V]: s_i = dsq_i exp(o0_i),
+// w[d]_i = W0 exp(o_{1+d}_i) for the edge (i, i + e_d) when it lies inside the
+// grid, else 0 (so the stencils need no separate edge test)
+__global__ void finalize_coef_kernel(int n0, int n1, int n2, const float* __restrict__ out,
+                                     const float* __restrict__ dsq, float* __restrict__ coef) {
+    const int64_t V = (int64_t)n0 * n1 * n2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i % n2), y = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
+        coef[i] = dsq[i] * expf(fminf(fmaxf(out[i], -OCLAMP), OCLAMP));
+        const bool in[3] = {x + 1 < n0, y + 1 < n1, z + 1 < n2};
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            coef[(1 + d) * V + i] = in[d] ? W0 * expf(fminf(fmaxf(out[(1 + d) * V + i], -OCLAMP), OCLAMP)) : 0.f;
+    }
 }
 
-// u = sy .* (G (sx .* y)):  u_i = sum_k g[k][i] x_{i + d_k}   (sx, sy may be null = 1)
-__global__ void __launch_bounds__(256) apply_g_kernel(int n0, int n1, int n2, const float* __restrict__ g,
-                                                      const float* __restrict__ y, const float* __restrict__ sx,
-                                                      const float* __restrict__ sy, float* __restrict__ u,
-                                                      const int* done) {
+// The 7-point graph stencil at cell i of grid (n0, n1, n2): for each axis d
+// the edge to i + e_d (weight w[d]_i, zero at the far face) and the edge to
+// i - e_d (weight w[d]_{i - e_d}, absent at the near face).
+struct Cell {
+    int64_t i, V;
+    int64_t st[3];
+    bool lo[3];
+};
+__device__ __forceinline__ Cell cell_at(int64_t i, int n0, int n1, int n2) {
+    Cell c;
+    c.i = i;
+    c.V = (int64_t)n0 * n1 * n2;
+    const int z = (int)(i % n2), y = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
+    c.st[0] = (int64_t)n1 * n2;
+    c.st[1] = n2;
+    c.st[2] = 1;
+    c.lo[0] = x > 0;
+    c.lo[1] = y > 0;
+    c.lo[2] = z > 0;
+    return c;
+}
+
+// ((I + L_w) (s .* v))_i for the cell c
+__device__ __forceinline__ float graph_apply(const Cell& c, const float* __restrict__ coef,
+                                             const float* __restrict__ v) {
+    const float yi = coef[c.i] * v[c.i];
+    float u = yi;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float wf = coef[(1 + d) * c.V + c.i];
+        if (wf != 0.f) {
+            const int64_t j = c.i + c.st[d];
+            u += wf * (yi - coef[j] * v[j]);
+        }
+        if (c.lo[d]) {
+            const int64_t j = c.i - c.st[d];
+            u += coef[(1 + d) * c.V + j] * (yi - coef[j] * v[j]);
+        }
+    }
+    return u;
+}
+
+// z = M r, optionally with the two FP64 dot partials of flexible CGLS
+// (z.r and z.(r - r_prev), one per block of 256 cells, like flex_dots_kernel)
+__global__ void __launch_bounds__(256) apply_m_kernel(int n0, int n1, int n2, const float* __restrict__ coef,
+                                                      const float* __restrict__ r, float* __restrict__ zo,
+                                                      const int* done, const float* __restrict__ r_prev,
+                                                      double* part_zr, double* part_fl) {
     if (done && *done) return;
     const int64_t V = (int64_t)n0 * n1 * n2;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
-        const int z = (int)(i % n2), yv = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            int dx, dy, dz;
-            g_offset(k, dx, dy, dz);
-            const int xx = x + dx, yy = yv + dy, zz = z + dz;
-            if ((unsigned)xx < (unsigned)n0 && (unsigned)yy < (unsigned)n1 && (unsigned)zz < (unsigned)n2) {
-                const int64_t j = ((int64_t)xx * n1 + yy) * n2 + zz;
-                acc += g[k * V + i] * y[j] * (sx ? sx[j] : 1.f);
-            }
-        }
-        u[i] = sy ? sy[i] * acc : acc;
-    }
-}
-
-// w = sy .* (G^T (sx .* v)):  w_j = sum_k g[k][j - d_k] x_{j - d_k}
-__global__ void __launch_bounds__(256) apply_gt_kernel(int n0, int n1, int n2, const float* __restrict__ g,
-                                                       const float* __restrict__ v, const float* __restrict__ sx,
-                                                       const float* __restrict__ sy, float* __restrict__ w,
-                                                       const int* done) {
-    if (done && *done) return;
-    const int64_t V = (int64_t)n0 * n1 * n2;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < V; j += (int64_t)gridDim.x * blockDim.x) {
-        const int z = (int)(j % n2), yv = (int)((j / n2) % n1), x = (int)(j / ((int64_t)n1 * n2));
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            int dx, dy, dz;
-            g_offset(k, dx, dy, dz);
-            const int xx = x - dx, yy = yv - dy, zz = z - dz;
-            if ((unsigned)xx < (unsigned)n0 && (unsigned)yy < (unsigned)n1 && (unsigned)zz < (unsigned)n2) {
-                const int64_t i = ((int64_t)xx * n1 + yy) * n2 + zz;
-                acc += g[k * V + i] * v[i] * (sx ? sx[i] : 1.f);
-            }
-        }
-        w[j] = sy ? sy[j] * acc : acc;
-    }
-}
-
-// dg[k][i] += a_i b_{i + d_k}
-__global__ void __launch_bounds__(256) coef_grad_kernel(int n0, int n1, int n2, const float* __restrict__ a,
-                                                        const float* __restrict__ b, float* __restrict__ dg) {
-    const int64_t V = (int64_t)n0 * n1 * n2;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
-        const int z = (int)(i % n2), yv = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
-        const float ai = a[i];
-#pragma unroll
-        for (int k = 0; k < NG; ++k) {
-            int dx, dy, dz;
-            g_offset(k, dx, dy, dz);
-            const int xx = x + dx, yy = yv + dy, zz = z + dz;
-            if ((unsigned)xx < (unsigned)n0 && (unsigned)yy < (unsigned)n1 && (unsigned)zz < (unsigned)n2) {
-                const int64_t j = ((int64_t)xx * n1 + yy) * n2 + zz;
-                dg[k * V + i] += ai * b[j];
-            }
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < V) {
+        const Cell c = cell_at(i, n0, n1, n2);
+        const float z = coef[i] * graph_apply(c, coef, r);
+        zo[i] = z;
+        if (part_zr) {
+            a = (double)z * (double)r[i];
+            b = (double)z * ((double)r[i] - (double)r_prev[i]);
         }
     }
-}
-
-// network output (SoA, NOUT rows) -> coefficients: diagonal exp(o), clamped
-// to exp(+-30); FP64 partial sums of the clamped exponent (= log G_ii)
-__global__ void finalize_g_kernel(int64_t V, float* g, double* part) {
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
-        const float o = fminf(fmaxf(g[i], -30.f), 30.f);
-        g[i] = expf(o);
-        acc += o;
-    }
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
+    if (!part_zr) return;
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && part) part[blockIdx.x] = red[0];
-}
-
-// out = x .* s ; out2 = s .* (t + lam * x .* s)   (two helpers of J)
-__global__ void scale_kernel(int64_t n, const float* x, const float* s, float* out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = x[i] * s[i];
-}
-__global__ void jfinish_kernel(int64_t n, const float* t, const float* xs, const float* s, float lam, float* out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = s[i] * (t[i] + lam * xs[i]);
-}
-
-// FP64 partial sums of a . b (one per block of 256)
-__global__ void dot_part_kernel(int64_t n, const float* a, const float* b, double* part) {
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        acc += (double)a[i] * (double)b[i];
-    __shared__ double red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        part_zr[blockIdx.x] = red[0][0];
+        part_fl[blockIdx.x] = red[1][0];
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// gradient of log K with respect to the network output, channels-last BF16:
-//   channel k >= 1: coef * dg[k][p];  channel 0: coef * dg[0][p] * G_pp - 2/V;
-//   channels >= NG: 0
-__global__ void out_grad_kernel(int64_t V, const float* dg, const float* g, float coef, float diag, bf16_t* d) {
+// dacc += d<g, M r>/d(s, w).  With y = s .* r, h = s .* g, P = I + L_w:
+//   d/ds_i    = g_i (P y)_i + (P h)_i r_i,
+//   d/dw[d]_i = (h_i - h_j)(y_i - y_j),  j = i + e_d.
+__global__ void __launch_bounds__(256) apply_m_grad_kernel(int n0, int n1, int n2, const float* __restrict__ coef,
+                                                           const float* __restrict__ r,
+                                                           const float* __restrict__ g, float* __restrict__ dacc) {
+    const int64_t V = (int64_t)n0 * n1 * n2;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < V; i += (int64_t)gridDim.x * blockDim.x) {
+        const Cell c = cell_at(i, n0, n1, n2);
+        const float si = coef[i];
+        dacc[i] += g[i] * graph_apply(c, coef, r) + graph_apply(c, coef, g) * r[i];
+        const float yi = si * r[i], hi = si * g[i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (coef[(1 + d) * V + i] == 0.f) continue;
+            const int64_t j = i + c.st[d];
+            dacc[(1 + d) * V + i] += (hi - coef[j] * g[j]) * (yi - coef[j] * r[j]);
+        }
+    }
+}
+
+// gradient with respect to the network output (channels-last BF16, NOUT
+// channels): chain rule through the exponentials (zero where the output was
+// clamped), channels >= NCOEF get 0
+__global__ void out_grad_kernel(int64_t V, const float* __restrict__ dacc, const float* __restrict__ coef,
+                                const float* __restrict__ out, float scale, bf16_t* __restrict__ d) {
     const int64_t n = V * NOUT;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const int c = (int)(k % NOUT);
+        const int ch = (int)(k % NOUT);
         const int64_t p = k / NOUT;
         float v = 0.f;
-        if (c == 0)
-            v = coef * dg[p] * g[p] - diag;
-        else if (c < NG)
-            v = coef * dg[c * V + p];
+        if (ch < NCOEF && fabsf(out[ch * V + p]) < OCLAMP) v = scale * dacc[ch * V + p] * coef[ch * V + p];
         d[k] = f2bf(v);
     }
 }
@@ -309,43 +304,40 @@ struct stp_lpc {
     bool train_ws = false;
     bf16_t *feat = nullptr, *a = nullptr, *pa = nullptr, *b = nullptr, *pb = nullptr, *c = nullptr, *cat2 = nullptr,
            *d = nullptr, *cat1 = nullptr, *e = nullptr;
-    float *g = nullptr, *dsq = nullptr, *tmp = nullptr;
+    float *out = nullptr, *coef = nullptr, *dsq = nullptr;
     double* nloc = nullptr;
     // training workspace
-    float *dg = nullptr, *de = nullptr, *dcat1 = nullptr, *dd = nullptr, *da = nullptr, *dcat2 = nullptr,
+    float *dacc = nullptr, *de = nullptr, *dcat1 = nullptr, *dd = nullptr, *da = nullptr, *dcat2 = nullptr,
           *dc = nullptr, *db = nullptr, *dpb = nullptr, *dpa = nullptr;
     bf16_t *dO = nullptr, *dpre = nullptr;
-    float *u = nullptr, *w1 = nullptr, *t = nullptr, *v = nullptr, *q = nullptr;
-    double* part = nullptr;   // 2 x PARTS FP64 block partial sums
-    int64_t qcap = 0;
     // state of the prepared preconditioner
     bool prepared = false;
+    bool dacc_live = false;   // dacc holds contributions since the last prepare / backward
 };
 
 namespace {
 
 void free_grid(stp_lpc* h) {
-    void* ps[] = {h->feat, h->a,  h->pa,    h->b,  h->pb, h->c,  h->cat2, h->d,   h->cat1, h->e,  h->g,
-                  h->dsq,  h->tmp, h->nloc, h->dg, h->de, h->dcat1, h->dd, h->da, h->dcat2, h->dc, h->db,
-                  h->dpb,  h->dpa, h->dO,   h->dpre, h->u, h->w1, h->t,  h->v,  h->part};
+    void* ps[] = {h->feat, h->a,    h->pa,   h->b,   h->pb,    h->c,   h->cat2, h->d,   h->cat1,
+                  h->e,    h->out,  h->coef, h->dsq, h->nloc,  h->dacc, h->de,   h->dcat1, h->dd,
+                  h->da,   h->dcat2, h->dc,  h->db,  h->dpb,   h->dpa,  h->dO,   h->dpre};
     for (void* p : ps)
         if (p) hipFree(p);
     h->feat = h->a = h->pa = h->b = h->pb = h->c = h->cat2 = h->d = h->cat1 = h->e = nullptr;
-    h->g = h->dsq = h->tmp = nullptr;
+    h->out = h->coef = h->dsq = nullptr;
     h->nloc = nullptr;
-    h->dg = h->de = h->dcat1 = h->dd = h->da = h->dcat2 = h->dc = h->db = h->dpb = h->dpa = nullptr;
+    h->dacc = h->de = h->dcat1 = h->dd = h->da = h->dcat2 = h->dc = h->db = h->dpb = h->dpa = nullptr;
     h->dO = h->dpre = nullptr;
-    h->u = h->w1 = h->t = h->v = nullptr;
-    h->part = nullptr;
     h->V = 0;
     h->n0 = h->n1 = h->n2 = 0;
     h->train_ws = false;
     h->prepared = false;
+    h->dacc_live = false;
 }
 
 void free_all(stp_lpc* h) {
     free_grid(h);
-    void* ps[] = {h->w32, h->wbf, h->wflip, h->grad, h->m1, h->m2, h->q};
+    void* ps[] = {h->w32, h->wbf, h->wflip, h->grad, h->m1, h->m2};
     for (void* p : ps)
         if (p) hipFree(p);
 }
@@ -373,13 +365,12 @@ int ensure_grid(stp_lpc* h, const int64_t* n, bool train) {
     STP_HIP(dalloc(&h->d, V2 * 32));
     STP_HIP(dalloc(&h->cat1, V * 48));
     STP_HIP(dalloc(&h->e, V * 16));
-    STP_HIP(dalloc(&h->g, V * NOUT));
-    STP_HIP(dalloc(&h->part, 2 * PARTS));
+    STP_HIP(dalloc(&h->out, V * NOUT));
+    STP_HIP(dalloc(&h->coef, V * NCOEF));
     STP_HIP(dalloc(&h->dsq, V));
-    STP_HIP(dalloc(&h->tmp, V));
     STP_HIP(dalloc(&h->nloc, V * NTAP));
     if (keep_train) {
-        STP_HIP(dalloc(&h->dg, V * NG));
+        STP_HIP(dalloc(&h->dacc, V * NCOEF));
         STP_HIP(dalloc(&h->de, V * 16));
         STP_HIP(dalloc(&h->dcat1, V * 48));
         STP_HIP(dalloc(&h->dd, V2 * 32));
@@ -391,10 +382,6 @@ int ensure_grid(stp_lpc* h, const int64_t* n, bool train) {
         STP_HIP(dalloc(&h->dpa, V2 * 16));
         STP_HIP(dalloc(&h->dO, V * NOUT));
         STP_HIP(dalloc(&h->dpre, V * 32));
-        STP_HIP(dalloc(&h->u, V));
-        STP_HIP(dalloc(&h->w1, V));
-        STP_HIP(dalloc(&h->t, V));
-        STP_HIP(dalloc(&h->v, V));
         h->train_ws = true;
     }
     return STP_OK;
@@ -438,9 +425,9 @@ int net_forward(stp_lpc* h, hipStream_t st) {
     STP_HIP(conv3d_fwd(shp(b0, b1, b2, D2), h->cat2, W(D2), 1, CONV_OUT_BF16, h->d, 32, 0, st));
     STP_HIP(upcat_fwd(a0, a1, a2, 32, 16, h->d, h->a, h->cat1, st));
     STP_HIP(conv3d_fwd(shp(a0, a1, a2, D1), h->cat1, W(D1), 1, CONV_OUT_BF16, h->e, 16, 0, st));
-    STP_HIP(conv3d_fwd(shp(a0, a1, a2, LO), h->e, W(LO), 0, CONV_OUT_F32_SOA, h->g, 0, 0, st));
-    // diagonal = exp(output 0); partial sums of log G_ii into part[PARTS ..]
-    hipLaunchKernelGGL(finalize_g_kernel, dim3(PARTS), dim3(256), 0, st, h->V, h->g, h->part + PARTS);
+    STP_HIP(conv3d_fwd(shp(a0, a1, a2, LO), h->e, W(LO), 0, CONV_OUT_F32_SOA, h->out, 0, 0, st));
+    hipLaunchKernelGGL(finalize_coef_kernel, dim3(gridn(h->V)), dim3(256), 0, st, a0, a1, a2, h->out, h->dsq,
+                       h->coef);
     STP_HIP(hipGetLastError());
     return STP_OK;
 }
@@ -493,85 +480,18 @@ int net_backward(stp_lpc* h, hipStream_t st) {
     return STP_OK;
 }
 
-// J x = dsq .* (A^T A (dsq .* x) + lam dsq .* x)
-int apply_j(stp_lpc* h, const stp_csr* A, double lam, const float* x, float* out, hipStream_t st) {
-    const int64_t V = h->V;
-    hipLaunchKernelGGL(scale_kernel, dim3(gridn(V)), dim3(256), 0, st, V, x, h->dsq, h->w1);
-    STP_HIP(hipGetLastError());
-    STP_HIP(stp::spmv_f32(A->W, A->rows, A->rp, A->ci, A->val, h->w1, h->q, nullptr, nullptr, st));
-    STP_HIP(stp::spmv_f32(A->WT, A->cols, A->trp, A->tci, A->tval, h->q, h->t, nullptr, nullptr, st));
-    hipLaunchKernelGGL(jfinish_kernel, dim3(gridn(V)), dim3(256), 0, st, V, h->t, h->w1, h->dsq, (float)lam, out);
-    STP_HIP(hipGetLastError());
-    return STP_OK;
-}
-
-// features + forward + loss + backward (gradient in h->grad).  Loss:
-//   log K~ = log(q / (P V)) - (2/V) sum_i log G_ii,   q = sum_y (G y)^T J (G y)
-// (the constant -(1/V) log det J is left out); dlogK/dg[k][i] =
-// (2/q) sum_y (J G y)_i y_{i+d_k}, and through G_ii = exp(o_i) the diagonal
-// output gets (2/q) sum_y (J G y)_i y_i G_ii - 2/V.
-int loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, const float* probes, int nprobe,
-              double* loss, hipStream_t st) {
-    int rc = check_system(A, n);
-    if (rc != STP_OK) return rc;
-    if (!probes || nprobe < 1) return fail(STP_ERR_INVALID, "learned preconditioner: no probe vectors");
-    if ((rc = ensure_grid(h, n, true)) != STP_OK) return rc;
-    if (h->qcap < A->rows) {
-        if (h->q) hipFree(h->q);
-        h->q = nullptr;
-        h->qcap = 0;
-        STP_HIP(dalloc(&h->q, A->rows));
-        h->qcap = A->rows;
-    }
-    h->prepared = false;
-    if ((rc = features(h, A, lam, nullptr, st)) != STP_OK) return rc;
-    if ((rc = net_forward(h, st)) != STP_OK) return rc;
-    const int64_t V = h->V;
-    const int n0 = h->n0, n1 = h->n1, n2 = h->n2;
-    STP_HIP(hipMemsetAsync(h->dg, 0, sizeof(float) * V * NG, st));
-    std::vector<double> parts((size_t)(nprobe + 1) * PARTS);
-    for (int k = 0; k < nprobe; ++k) {
-        const float* y = probes + (int64_t)k * V;
-        hipLaunchKernelGGL(apply_g_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->g, y, nullptr, nullptr,
-                           h->u, nullptr);
-        STP_HIP(hipGetLastError());
-        if ((rc = apply_j(h, A, lam, h->u, h->v, st)) != STP_OK) return rc;
-        hipLaunchKernelGGL(dot_part_kernel, dim3(PARTS), dim3(256), 0, st, V, h->u, h->v, h->part);
-        hipLaunchKernelGGL(coef_grad_kernel, dim3(gridn(V)), dim3(256), 0, st, n0, n1, n2, h->v, y, h->dg);
-        STP_HIP(hipGetLastError());
-        STP_HIP(hipMemcpyAsync(parts.data() + (size_t)k * PARTS, h->part, sizeof(double) * PARTS,
-                               hipMemcpyDeviceToHost, st));
-    }
-    STP_HIP(hipMemcpyAsync(parts.data() + (size_t)nprobe * PARTS, h->part + PARTS, sizeof(double) * PARTS,
-                           hipMemcpyDeviceToHost, st));
-    STP_HIP(hipStreamSynchronize(st));
-    double q = 0.0, logdiag = 0.0;
-    for (int k = 0; k < nprobe; ++k)
-        for (int j = 0; j < PARTS; ++j) q += parts[(size_t)k * PARTS + j];
-    for (int j = 0; j < PARTS; ++j) logdiag += parts[(size_t)nprobe * PARTS + j];
-    if (!(q > 0.0) || !std::isfinite(q))
-        return fail(STP_ERR_INVALID, "learned preconditioner: non-positive or non-finite trace estimate");
-    if (loss) *loss = std::log(q / ((double)nprobe * (double)V)) - 2.0 * logdiag / (double)V;
-    hipLaunchKernelGGL(out_grad_kernel, dim3(gridn(V * NOUT)), dim3(256), 0, st, V, h->dg, h->g, (float)(2.0 / q),
-                       (float)(2.0 / (double)V), h->dO);
-    STP_HIP(hipGetLastError());
-    if ((rc = net_backward(h, st)) != STP_OK) return rc;
-    STP_HIP(hipStreamSynchronize(st));
-    return STP_OK;
-}
-
 }  // namespace
 
 namespace stp {
-int lpc_apply(void* handle, const float* s, float* z, const int* done, hipStream_t st) {
+int lpc_apply(void* handle, const float* s, float* z, const int* done, const float* s_prev, double* part_zs,
+              double* part_fl, hipStream_t st) {
     stp_lpc* h = static_cast<stp_lpc*>(handle);
     if (!h || !h->prepared) return fail(STP_ERR_STATE, "learned preconditioner: not prepared for a system");
+    if ((part_zs || part_fl) && !(part_zs && part_fl && s_prev))
+        return fail(STP_ERR_INVALID, "learned preconditioner: incomplete dot-product outputs");
     const int64_t V = h->V;
-    // tmp = G^T (dsq .* s) ; z = dsq .* (G tmp)
-    hipLaunchKernelGGL(apply_gt_kernel, dim3(gridn(V)), dim3(256), 0, st, h->n0, h->n1, h->n2, h->g, s, h->dsq,
-                       nullptr, h->tmp, done);
-    hipLaunchKernelGGL(apply_g_kernel, dim3(gridn(V)), dim3(256), 0, st, h->n0, h->n1, h->n2, h->g, h->tmp, nullptr,
-                       h->dsq, z, done);
+    hipLaunchKernelGGL(apply_m_kernel, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, h->n0, h->n1, h->n2,
+                       h->coef, s, z, done, s_prev, part_zs, part_fl);
     STP_HIP(hipGetLastError());
     return STP_OK;
 }
@@ -600,7 +520,7 @@ int stp_lpc_create(uint64_t seed, stp_lpc** out) {
         return stp::hip_fail(e, "stp_lpc_create");
     }
     // He-uniform initialisation over the real fan-in; the output layer starts
-    // at zero (G = I, i.e. Jacobi), padded input/output channels stay zero.
+    // at zero (M = D^-1/2 (I + W0 L) D^-1/2), padded input channels stay zero.
     std::vector<float> w(P, 0.f);
     uint64_t s = seed;
     for (int l = 0; l < 6; ++l) {
@@ -670,56 +590,84 @@ int stp_lpc_features(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam,
     return STP_OK;
 }
 
-int stp_lpc_prepare(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, void* stream) {
+static int prepare(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, bool train, void* stream) {
     if (!h) return fail(STP_ERR_INVALID, "stp_lpc_prepare: null handle");
     int rc = check_system(A, n);
     if (rc != STP_OK) return rc;
-    if ((rc = ensure_grid(h, n, false)) != STP_OK) return rc;
+    if ((rc = ensure_grid(h, n, train)) != STP_OK) return rc;
     hipStream_t st = S(stream);
     h->prepared = false;
     if ((rc = features(h, A, lam, nullptr, st)) != STP_OK) return rc;
     if ((rc = net_forward(h, st)) != STP_OK) return rc;
     h->prepared = true;
+    h->dacc_live = false;
     return STP_OK;
 }
 
-int stp_lpc_coefficients(const stp_lpc* h, float* g_dev, void* stream) {
-    if (!h || !g_dev) return fail(STP_ERR_INVALID, "stp_lpc_coefficients: null argument");
+int stp_lpc_prepare(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, void* stream) {
+    return prepare(h, A, n, lam, false, stream);
+}
+
+int stp_lpc_prepare_train(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, void* stream) {
+    return prepare(h, A, n, lam, true, stream);
+}
+
+int stp_lpc_coefficients(const stp_lpc* h, float* coef_dev, void* stream) {
+    if (!h || !coef_dev) return fail(STP_ERR_INVALID, "stp_lpc_coefficients: null argument");
     if (!h->prepared) return fail(STP_ERR_STATE, "stp_lpc_coefficients: not prepared for a system");
-    STP_HIP(hipMemcpyAsync(g_dev, h->g, sizeof(float) * NG * h->V, hipMemcpyDeviceToDevice, S(stream)));
+    STP_HIP(hipMemcpyAsync(coef_dev, h->coef, sizeof(float) * NCOEF * h->V, hipMemcpyDeviceToDevice, S(stream)));
     return STP_OK;
 }
 
 int stp_lpc_apply(const stp_lpc* h, const float* s_dev, float* z_dev, void* stream) {
     if (!h || !s_dev || !z_dev) return fail(STP_ERR_INVALID, "stp_lpc_apply: null argument");
-    return stp::lpc_apply(const_cast<stp_lpc*>(h), s_dev, z_dev, nullptr, S(stream));
+    return stp::lpc_apply(const_cast<stp_lpc*>(h), s_dev, z_dev, nullptr, nullptr, nullptr, nullptr, S(stream));
 }
 
-int stp_lpc_loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, const float* probes_dev,
-                      int32_t nprobe, double* loss, float* grad_dev, void* stream) {
-    if (!h) return fail(STP_ERR_INVALID, "stp_lpc_loss_grad: null handle");
+int stp_lpc_backward_accumulate(stp_lpc* h, const float* r_dev, const float* g_dev, void* stream) {
+    if (!h || !r_dev || !g_dev) return fail(STP_ERR_INVALID, "stp_lpc_backward_accumulate: null argument");
+    if (!h->prepared) return fail(STP_ERR_STATE, "stp_lpc_backward_accumulate: not prepared for a system");
     hipStream_t st = S(stream);
-    int rc = loss_grad(h, A, n, lam, probes_dev, nprobe, loss, st);
-    if (rc != STP_OK) return rc;
-    if (grad_dev) {
-        STP_HIP(hipMemcpyAsync(grad_dev, h->grad, sizeof(float) * total_params(), hipMemcpyDeviceToDevice, st));
-        STP_HIP(hipStreamSynchronize(st));
+    if (!h->train_ws)
+        return fail(STP_ERR_STATE, "stp_lpc_backward_accumulate: system prepared without the training workspace "
+                                   "(use stp_lpc_prepare_train)");
+    if (!h->dacc_live) {
+        STP_HIP(hipMemsetAsync(h->dacc, 0, sizeof(float) * NCOEF * h->V, st));
+        h->dacc_live = true;
     }
+    hipLaunchKernelGGL(apply_m_grad_kernel, dim3(gridn(h->V)), dim3(256), 0, st, h->n0, h->n1, h->n2, h->coef, r_dev,
+                       g_dev, h->dacc);
+    STP_HIP(hipGetLastError());
     return STP_OK;
 }
 
-int stp_lpc_train_step(stp_lpc* h, const stp_csr* A, const int64_t* n, double lam, const float* probes_dev,
-                       int32_t nprobe, double lr, double* loss, void* stream) {
-    if (!h || !(lr > 0)) return fail(STP_ERR_INVALID, "stp_lpc_train_step: bad argument");
+int stp_lpc_backward(stp_lpc* h, double scale, float* grad_dev, void* stream) {
+    if (!h) return fail(STP_ERR_INVALID, "stp_lpc_backward: null handle");
+    if (!h->prepared || !h->train_ws || !h->dacc_live)
+        return fail(STP_ERR_STATE, "stp_lpc_backward: no accumulated output gradient");
     hipStream_t st = S(stream);
-    int rc = loss_grad(h, A, n, lam, probes_dev, nprobe, loss, st);
+    hipLaunchKernelGGL(out_grad_kernel, dim3(gridn(h->V * NOUT)), dim3(256), 0, st, h->V, h->dacc, h->coef, h->out,
+                       (float)scale, h->dO);
+    STP_HIP(hipGetLastError());
+    const int rc = net_backward(h, st);
     if (rc != STP_OK) return rc;
+    h->dacc_live = false;
+    if (grad_dev)
+        STP_HIP(hipMemcpyAsync(grad_dev, h->grad, sizeof(float) * total_params(), hipMemcpyDeviceToDevice, st));
+    STP_HIP(hipStreamSynchronize(st));
+    return STP_OK;
+}
+
+int stp_lpc_adam_step(stp_lpc* h, double lr, void* stream) {
+    if (!h || !(lr > 0)) return fail(STP_ERR_INVALID, "stp_lpc_adam_step: bad argument");
+    hipStream_t st = S(stream);
     const float b1 = 0.9f, b2 = 0.999f;
     h->steps += 1;
     const float c1 = 1.0f / (1.0f - std::pow(b1, (float)h->steps));
     const float c2 = 1.0f / (1.0f - std::pow(b2, (float)h->steps));
     STP_HIP(stp::adam_step(total_params(), h->w32, h->grad, h->m1, h->m2, (float)lr, b1, b2, 1e-8f, c1, c2, h->wbf, st));
     STP_HIP(hipStreamSynchronize(st));
+    h->prepared = false;   // the coefficients no longer match the weights
     return STP_OK;
 }
 

@@ -280,10 +280,9 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z, h->part2,
                                      h->part3, nullptr, st));
         if (flex) {
-            int rc = stp::lpc_apply(precond_handle, h->s, h->z, nullptr, st);
-            if (rc != STP_OK) return rc;
             STP_HIP(hipMemsetAsync(h->sp, 0, sizeof(float) * cols, st));
-            STP_HIP(stp::launch_flex_dots(nullptr, cols, h->z, h->s, h->sp, h->part4, h->part5, st));
+            int rc = stp::lpc_apply(precond_handle, h->s, h->z, nullptr, h->sp, h->part4, h->part5, st);
+            if (rc != STP_OK) return rc;
             STP_HIP(stp::launch_cgls_init(h->sc, h->part4, nbc, h->part3, nbs, set_s0, st));
         } else {
             STP_HIP(stp::launch_cgls_init(h->sc, h->part2, nbs, h->part3, nbs, set_s0, st));
@@ -312,9 +311,8 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
             STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z,
                                          h->part2, h->part3, done, st));
             if (flex) {
-                rc = stp::lpc_apply(precond_handle, h->s, h->z, done, st);
+                rc = stp::lpc_apply(precond_handle, h->s, h->z, done, h->sp, h->part4, h->part5, st);
                 if (rc != STP_OK) return rc;
-                STP_HIP(stp::launch_flex_dots(done, cols, h->z, h->s, h->sp, h->part4, h->part5, st));
                 STP_HIP(stp::launch_cgls_beta(h->sc, h->part4, nbc, h->part3, nbs, h->part5, nbc, tol, history, it,
                                               st));
             } else {

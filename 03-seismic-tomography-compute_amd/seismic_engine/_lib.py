@@ -91,10 +91,12 @@ SIGNATURES: dict[str, list] = {
     "stp_lpc_set_params": [_vp, _vp, _i64],
     "stp_lpc_features": [_vp, _vp, _vp, _f64, _vp, _vp, _vp],
     "stp_lpc_prepare": [_vp, _vp, _vp, _f64, _vp],
+    "stp_lpc_prepare_train": [_vp, _vp, _vp, _f64, _vp],
     "stp_lpc_coefficients": [_vp, _vp, _vp],
     "stp_lpc_apply": [_vp, _vp, _vp, _vp],
-    "stp_lpc_loss_grad": [_vp, _vp, _vp, _f64, _vp, _i32, C.POINTER(_f64), _vp, _vp],
-    "stp_lpc_train_step": [_vp, _vp, _vp, _f64, _vp, _i32, _f64, C.POINTER(_f64), _vp],
+    "stp_lpc_backward_accumulate": [_vp, _vp, _vp, _vp],
+    "stp_lpc_backward": [_vp, _f64, _vp, _vp],
+    "stp_lpc_adam_step": [_vp, _f64, _vp],
     "stp_conv3d_bf16": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _i32, _vp, _vp],
     "stp_conv3d_bf16_bwd": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
 }

@@ -5,10 +5,13 @@ The reference preconditions CG with Jacobi only
 (src/cpp/sparse/sparse_ops.cpp:52-113); that stays the fallback and the judge.
 Here a 3-level 3-D convolutional network (16/32/64 channels, 3x3x3, ReLU,
 BF16 on MFMA with FP32 accumulation) maps the Jacobi-scaled local couplings of
-N = A^T A + lam I to an upper-triangular 63-tap factor G, and CGLS runs with
-M = D^-1/2 G G^T D^-1/2.  Untrained, G = I and M is Jacobi.  Training runs on
-the GPU through the library's own forward and backward kernels; it minimises
-Kaporin's condition number of G^T J G, estimated from Rademacher probes.
+N = A^T A + lam I to a per-cell scale and per-edge weights, and CGLS runs with
+M = S (I + L_w) S (S = diag(exp(o0)/sqrt(N_ii)), L_w the weighted 7-point grid
+Laplacian): symmetric positive definite for any weights.  Untrained it is
+D^-1/2 (I + L) D^-1/2.  Training runs on the GPU through the library's own
+forward and backward kernels: preconditioned CG is unrolled on training
+systems, and the energy-norm errors of its iterates are back-propagated into
+the network (`unrolled_cg_loss`).
 """
 from __future__ import annotations
 
@@ -20,29 +23,14 @@ import numpy as np
 
 from ._lib import call, lib, ptr, stream_handle
 
-NG = 63            # taps of G
+NCOEF = 4          # s, w_x, w_y, w_z per cell
 NFEAT = 32         # network input channels
-LAYERS = ((32, 16), (16, 32), (32, 64), (96, 32), (48, 16), (16, 64))   # (cin, cout): e1 e2 e3 d2 d1 o
+LAYERS = ((32, 16), (16, 32), (32, 64), (96, 32), (48, 16), (16, 16))   # (cin, cout): e1 e2 e3 d2 d1 o
 
 
 def _torch():
     import torch
     return torch
-
-
-def g_offsets() -> np.ndarray:
-    """(63, 3) offsets d_k of the taps of G: the (62 + k)-th offset of
-    [-2,2]^3 in lexicographic (x, y, z) order; row 0 is the diagonal."""
-    o = np.array([(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)], dtype=np.int64)
-    return o[62:]
-
-
-def rademacher(nprobe: int, ncell: int, seed: int, device=None):
-    """Probe vectors with entries +-1 (E[y y^T] = I), [nprobe, ncell] FP32."""
-    torch = _torch()
-    g = torch.Generator(device="cpu").manual_seed(int(seed))
-    y = torch.randint(0, 2, (nprobe, ncell), generator=g, dtype=torch.int8).to(torch.float32) * 2 - 1
-    return y.to(device) if device is not None else y
 
 
 class LearnedPreconditioner:
@@ -105,45 +93,115 @@ class LearnedPreconditioner:
         call("stp_lpc_features", self.handle, A._h, n.ctypes.data, C.c_double(lam), ptr(f), ptr(d), stream_handle())
         return f, d
 
-    def prepare(self, A, grid, lam: float) -> None:
+    def prepare(self, A, grid, lam: float, train: bool = False) -> None:
+        """Features + network for system A: M is ready for `apply` and for
+        CGLS with precond="learned".  train=True keeps what the backward pass
+        needs."""
         n = self._grid(grid)
-        call("stp_lpc_prepare", self.handle, A._h, n.ctypes.data, C.c_double(lam), stream_handle())
+        call("stp_lpc_prepare_train" if train else "stp_lpc_prepare", self.handle, A._h, n.ctypes.data,
+             C.c_double(lam), stream_handle())
         self._prepared_cells = int(np.prod(n))
 
     def coefficients(self):
-        """[63, cell] coefficients of G for the prepared system."""
+        """[4, cell] coefficients of M for the prepared system: s, w_x, w_y, w_z."""
         torch = _torch()
-        g = torch.empty(NG, self._prepared_cells, dtype=torch.float32, device="cuda")
+        g = torch.empty(NCOEF, self._prepared_cells, dtype=torch.float32, device="cuda")
         call("stp_lpc_coefficients", self.handle, ptr(g), stream_handle())
         return g
 
     def apply(self, s):
-        """z = M s for the prepared system."""
+        """z = M s for the prepared system (FP32)."""
         torch = _torch()
         s = s.to(device="cuda", dtype=torch.float32).contiguous()
         z = torch.empty_like(s)
         call("stp_lpc_apply", self.handle, ptr(s), ptr(z), stream_handle())
         return z
 
-    def loss_grad(self, A, grid, lam: float, probes):
+    def backward_accumulate(self, r, g) -> None:
+        """Add d<g, M r>/d(network output) to the handle's output gradient."""
         torch = _torch()
-        n = self._grid(grid)
-        probes = probes.to(device="cuda", dtype=torch.float32).contiguous()
-        loss = C.c_double()
-        g = torch.empty(self.num_params, dtype=torch.float32, device="cuda")
-        call("stp_lpc_loss_grad", self.handle, A._h, n.ctypes.data, C.c_double(lam), ptr(probes),
-             int(probes.shape[0]), C.byref(loss), ptr(g), stream_handle())
-        return loss.value, g
+        r = r.to(device="cuda", dtype=torch.float32).contiguous()
+        g = g.to(device="cuda", dtype=torch.float32).contiguous()
+        call("stp_lpc_backward_accumulate", self.handle, ptr(r), ptr(g), stream_handle())
 
-    def train_step(self, A, grid, lam: float, probes, lr: float) -> float:
+    def backward(self, scale: float = 1.0, want_grad: bool = False):
+        """Network backward of the accumulated output gradient (times scale);
+        returns the parameter gradient when asked."""
         torch = _torch()
-        n = self._grid(grid)
-        probes = probes.to(device="cuda", dtype=torch.float32).contiguous()
-        loss = C.c_double()
-        call("stp_lpc_train_step", self.handle, A._h, n.ctypes.data, C.c_double(lam), ptr(probes),
-             int(probes.shape[0]), C.c_double(lr), C.byref(loss), stream_handle())
+        g = torch.empty(self.num_params, dtype=torch.float32, device="cuda") if want_grad else None
+        call("stp_lpc_backward", self.handle, C.c_double(scale), ptr(g), stream_handle())
+        return g
+
+    def adam_step(self, lr: float) -> None:
+        call("stp_lpc_adam_step", self.handle, C.c_double(lr), stream_handle())
         self.steps += 1
-        return loss.value
+
+
+def _autograd_ops():
+    """torch.autograd Functions over the library: the normal operator
+    N x = A^T A x + lam x (FP64 SpMV; N is symmetric, so its backward is
+    itself) and the preconditioner z = M r (FP32 stencil; M is symmetric, and
+    the backward also accumulates the gradient of the network output)."""
+    torch = _torch()
+
+    class NormalOp(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, x, A, lam):
+            ctx.A, ctx.lam = A, lam
+            return A.matvec(A.matvec(x), transpose=True) + lam * x
+
+        @staticmethod
+        def backward(ctx, g):
+            A, lam = ctx.A, ctx.lam
+            return A.matvec(A.matvec(g.contiguous()), transpose=True) + lam * g, None, None
+
+    class PrecondOp(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, r, net, token):
+            ctx.net = net
+            ctx.save_for_backward(r)
+            return net.apply(r).to(r.dtype)
+
+        @staticmethod
+        def backward(ctx, g):
+            (r,) = ctx.saved_tensors
+            ctx.net.backward_accumulate(r, g)
+            return ctx.net.apply(g).to(g.dtype), None, torch.zeros((), dtype=g.dtype, device=g.device)
+
+    return NormalOp, PrecondOp
+
+
+def unrolled_cg_loss(net, A, rhs, x_star, lam: float, k1: int, k2: int):
+    """Preconditioned CG on N x = rhs from x = 0 for k2 iterations (FP64
+    vectors), with M the prepared network; the loss is the mean over k in
+    [k1, k2] of 0.5 log(|e_k|_N^2 / |e_0|_N^2), e_k = x_star - x_k, where
+    |e_k|_N^2 = e_k . r_k (N e_k = r_k when x_star solves the system).  The
+    result is a differentiable torch scalar: loss.backward() accumulates the
+    output gradient in `net` (call net.backward() afterwards)."""
+    torch = _torch()
+    NormalOp, PrecondOp = _autograd_ops()
+    token = torch.zeros((), dtype=torch.float64, device="cuda", requires_grad=True)
+    x = torch.zeros_like(rhs)
+    r = rhs.clone()
+    e0 = float(x_star @ rhs)
+    z = PrecondOp.apply(r, net, token)
+    p = z
+    rz = r @ z
+    terms = []
+    for k in range(k2):
+        q = NormalOp.apply(p, A, lam)
+        alpha = rz / (p @ q)
+        x = x + alpha * p
+        r = r - alpha * q
+        if k + 1 >= k1:
+            terms.append(0.5 * torch.log(torch.clamp((x_star - x) @ r, min=1e-300) / e0))
+        if k + 1 == k2:
+            break
+        z = PrecondOp.apply(r, net, token)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return torch.stack(terms).mean()
 
 
 # -------------------------------------------------------------------- single layers (tests, tuning)
@@ -196,27 +254,54 @@ def tomography_system(grid, nrays: int, seed: int, h: float = 1.0):
     return DeviceCSR.from_rays(pts, tuple(grid), h)
 
 
-def train(net: LearnedPreconditioner, systems, n_systems: int, lam: float = 1e-2, steps_per_system: int = 2,
-          nprobe: int = 4, lr: float = 2e-3, lr_final: float = 2e-4, seed: int = 0, log_every: int = 0) -> TrainLog:
-    """Adam on log K over an iterable of n_systems (grid, matrix) systems
-    (consumed lazily, so each matrix can be freed after use),
-    `steps_per_system` steps each with fresh Rademacher probes; the learning
-    rate decays geometrically from lr to lr_final."""
+@dataclass
+class TrainSystem:
+    """One training system of the tomography recipe: the device matrix, its
+    grid, rhs = A^T b and a tightly converged solution."""
+    grid: tuple
+    A: object
+    b: object
+    rhs: object
+    x_star: object
+
+
+def make_system(grid, nrays: int, seed: int, lam: float = 1e-2, tol: float = 1e-9) -> TrainSystem:
+    """Rays between seeded points on opposite faces (`tomography_system`),
+    slowness of a seeded smooth random velocity model, b = A s, and x_star
+    from Jacobi-CGLS to `tol`."""
     torch = _torch()
-    total = max(1, n_systems * steps_per_system)
+    from . import synthetic
+    A = tomography_system(grid, nrays, seed)
+    s = torch.as_tensor(1.0 / synthetic.smooth_random_velocity(*grid, seed=seed + 1, sigma=20.0 * grid[0] / 512 * 4),
+                        dtype=torch.float64, device="cuda").reshape(-1)
+    b = A.matvec(s)
+    x_star, _, _ = A.cgls(b, lam, tol, 4000, "jacobi")
+    return TrainSystem(tuple(grid), A, b, A.matvec(b, transpose=True), x_star)
+
+
+def train(net: LearnedPreconditioner, systems, lam: float = 1e-2, k_window=(20, 100), steps_per_system: int = 1,
+          lr: float = 1e-3, lr_final: float = 1e-4, n_steps: int | None = None, log_every: int = 0) -> TrainLog:
+    """Adam on the unrolled-CG loss over an iterable of TrainSystem (consumed
+    lazily); the learning rate decays geometrically from lr to lr_final over
+    n_steps (default: len(systems) * steps_per_system)."""
+    torch = _torch()
+    systems = list(systems) if n_steps is None else systems
+    total = n_steps or max(1, len(systems) * steps_per_system)
     log = TrainLog()
     t0 = time.perf_counter()
     k = 0
-    for si, (grid, A) in enumerate(systems):
-        V = int(np.prod(grid))
+    for si, sysm in enumerate(systems):
         for _ in range(steps_per_system):
             rate = lr * (lr_final / lr) ** (k / max(1, total - 1))
-            y = rademacher(nprobe, V, seed * 1_000_003 + k, device="cuda")
-            loss = net.train_step(A, grid, lam, y, rate)
-            log.losses.append(loss)
+            net.prepare(sysm.A, sysm.grid, lam, train=True)
+            loss = unrolled_cg_loss(net, sysm.A, sysm.rhs, sysm.x_star, lam, *k_window)
+            loss.backward()
+            net.backward()
+            net.adam_step(rate)
+            log.losses.append(float(loss.detach()))
             k += 1
             if log_every and k % log_every == 0:
-                print(f"[train] step {k}/{total} system {si} loss {loss:.4f}", flush=True)
+                print(f"[train] step {k}/{total} grid {sysm.grid} loss {log.losses[-1]:.4f}", flush=True)
         log.systems += 1
     torch.cuda.synchronize()
     log.seconds = time.perf_counter() - t0
@@ -228,45 +313,57 @@ def train(net: LearnedPreconditioner, systems, n_systems: int, lam: float = 1e-2
 DEFAULT_WEIGHTS = __import__("pathlib").Path(__file__).resolve().parent / "data" / "lpc_tomography.npz"
 
 
-def solve_iterations(net, grid, nrays: int, seed: int, lam: float = 1e-2, tol: float = 1e-6) -> tuple[int, int]:
-    """(Jacobi, learned) CGLS iterations to tol on one system of the recipe."""
-    torch = _torch()
-    from . import synthetic
-    A = tomography_system(grid, nrays, seed)
-    n = grid[0]
-    s = torch.as_tensor(1.0 / synthetic.smooth_random_velocity(*grid, seed=seed + 1, sigma=20.0 * n / 512 * 4),
-                        dtype=torch.float64, device="cuda").reshape(-1)
-    b = A.matvec(s)
-    _, ij, _ = A.cgls(b, lam, tol, 20000, "jacobi")
-    net.prepare(A, grid, lam)
-    _, il, _ = A.cgls(b, lam, tol, 20000, "learned", network=net)
+def solve_iterations(net, sysm: TrainSystem, lam: float = 1e-2, tol: float = 1e-6) -> tuple[int, int]:
+    """(Jacobi, learned) CGLS iterations to tol on one system."""
+    _, ij, _ = sysm.A.cgls(sysm.b, lam, tol, 20000, "jacobi")
+    net.prepare(sysm.A, sysm.grid, lam)
+    _, il, _ = sysm.A.cgls(sysm.b, lam, tol, 20000, "learned", network=net)
     return ij.iterations, il.iterations
 
 
-def train_recipe(seed: int = 0, n_systems: int = 400, grid=(32, 32, 32), nrays: int = 50_000,
-                 val_every: int = 50, val=((64, 64, 64), 200_000, 90_000), log_every: int = 0):
-    """The training run behind the default weights: Adam on log K over
-    `n_systems` seeded systems of the tomography recipe at `grid` (the
-    200k-ray 64^3 density per cell), 2 steps per system with 4 Rademacher
-    probes; every `val_every` systems the validation system `val` (grid,
-    rays, seed; seeds disjoint from training and from the benchmark's) is
-    solved and the weights with the fewest learned-CGLS iterations are kept."""
+# (grid, rays) of the training mix: the 200k-ray 64^3 density of rays per
+# cell face, at three grid sizes
+TRAIN_MIX = (((32, 32, 32), 50_000), ((48, 48, 48), 112_000), ((64, 64, 64), 200_000))
+VALIDATION = (((64, 64, 64), 200_000, 90_000), ((48, 48, 48), 112_000, 90_002))
+
+
+def train_recipe(seed: int = 0, n_systems: int = 120, val_every: int = 20, k_window=(20, 100), lr: float = 1e-3,
+                 lr_final: float = 1e-4, log_every: int = 0):
+    """The training run behind the default weights: Adam on the unrolled-CG
+    loss over `n_systems` seeded systems cycling through TRAIN_MIX (seeds
+    from 10_000 on, disjoint from the validation seeds and from the
+    benchmark's), one step per system; every `val_every` systems the
+    VALIDATION systems are solved with learned CGLS and the weights with the
+    fewest iterations in total are kept (the untrained network included)."""
     net = LearnedPreconditioner(seed=seed)
-    best = (solve_iterations(net, *val)[1], net.get_params(), 0)
+    lam = 1e-2
+    val = [make_system(g, r, sd, lam) for g, r, sd in VALIDATION]
+
+    def score():
+        return sum(solve_iterations(net, v, lam)[1] for v in val)
+
+    best = (score(), net.get_params(), 0)
     history = [(0, best[0])]
     done = 0
+    t0 = time.perf_counter()
     while done < n_systems:
         k = min(val_every, n_systems - done)
-        train(net, ((grid, tomography_system(grid, nrays, seed=s)) for s in range(done, done + k)), n_systems=k,
-              steps_per_system=2, nprobe=4, lr=2e-3 * (0.3 ** (done / max(1, n_systems))),
-              lr_final=2e-3 * (0.3 ** ((done + k) / max(1, n_systems))), seed=seed + done, log_every=log_every)
+        systems = (make_system(*TRAIN_MIX[i % len(TRAIN_MIX)], seed=10_000 + 2 * i, lam=lam)
+                   for i in range(done, done + k))
+        lr0 = lr * (lr_final / lr) ** (done / n_systems)
+        lr1 = lr * (lr_final / lr) ** ((done + k) / n_systems)
+        train(net, systems, lam, k_window, 1, lr0, lr1, n_steps=k, log_every=log_every)
         done += k
-        it = solve_iterations(net, *val)[1]
+        it = score()
         history.append((done, it))
+        if log_every:
+            print(f"[train] after {done} systems: validation iterations {it} (best {best[0]})", flush=True)
         if it < best[0]:
             best = (it, net.get_params(), done)
     net.set_params(best[1])
-    return net, {"validation": history, "kept_after_systems": best[2], "validation_iterations": best[0]}
+    jac = sum(solve_iterations(net, v, lam)[0] for v in val)
+    return net, {"validation_iterations": history, "kept_after_systems": best[2], "best_iterations": best[0],
+                 "jacobi_iterations": jac, "seconds": round(time.perf_counter() - t0, 1)}
 
 
 if __name__ == "__main__":
@@ -274,10 +371,11 @@ if __name__ == "__main__":
     import json
     ap = argparse.ArgumentParser(description="train the default learned preconditioner on the GPU")
     ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
-    ap.add_argument("--systems", type=int, default=400)
+    ap.add_argument("--systems", type=int, default=120)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
-    net, info = train_recipe(seed=args.seed, n_systems=args.systems, log_every=100)
+    net, info = train_recipe(seed=args.seed, n_systems=args.systems, lr=args.lr, log_every=10)
     __import__("pathlib").Path(args.out).parent.mkdir(parents=True, exist_ok=True)
     net.save(args.out)
     print(json.dumps(info), flush=True)

@@ -116,7 +116,11 @@ class DeviceCSR:
         return (x, info, hist[: info.iterations] if history else None)
 
     def cgls(self, b, lam: float, tol: float = 1e-6, max_iter: int = 10000, precond: str = "none",
-             network=None, history=False):
+             network=None, history=False, fallback: bool = True):
+        """CGLS on (A^T A + lam I) x = A^T b.  precond "none", "jacobi" or
+        "learned" (a prepared LearnedPreconditioner in `network`); a learned
+        solve that does not reach tol in max_iter is redone with Jacobi when
+        `fallback` (info.fallback tells)."""
         torch = _torch()
         kind = {"none": 0, "jacobi": 1, "learned": 2}[precond]
         bb = _as_dev(b, torch.float64)
@@ -126,6 +130,12 @@ class DeviceCSR:
         hnet = network.handle if network is not None else None
         call("stp_cgls", self._h, ptr(bb), C.c_double(lam), C.c_double(tol), int(max_iter), kind, hnet, ptr(x),
              ptr(hist), C.byref(info), stream_handle())
+        info.fallback = False
+        if kind == 2 and fallback and not info.converged:
+            # Jacobi-CGLS is the fallback of the learned preconditioner
+            x, info, hist = self.cgls(b, lam, tol, max_iter, "jacobi", history=history)
+            info.fallback = True
+            return x, info, hist
         return x, info, (hist[: info.iterations] if history else None)
 
 

@@ -255,15 +255,16 @@ int stp_adjoint_gradient_f64(int64_t n, const double* fwd_dev, const double* adj
  * (16/32/64 channels, 3x3x3, ReLU; BF16 MFMA, FP32 accumulation) reads the
  * Jacobi-scaled couplings of N = A^T A + lambda I to the 26 neighbours of each
  * cell of an n0 x n1 x n2 grid (A's columns, x-major; each dimension a
- * multiple of 4) and outputs row i of an upper-triangular factor G: 63 taps,
- * offsets in [-2,2]^3 lexicographically >= (0,0,0), the diagonal first and
- * positive (exp of the network output).  The preconditioner is
- * M = D^-1/2 G G^T D^-1/2 with D = diag(N): symmetric positive definite for
- * any weights.  Training minimises Kaporin's condition number
- * log K = log(tr(G^T J G)/n) - (2/n) sum log G_ii (+ const) of
- * J = D^-1/2 N D^-1/2, the trace estimated from probe vectors with
- * E[y y^T] = I (Rademacher +-1).  Untrained (zero output layer), G = I and
- * M is Jacobi.  A handle holds one prepared system at a time. */
+ * multiple of 4) and outputs, per cell, a log scale and the log weights of
+ * the three grid edges to i + e_x, i + e_y, i + e_z.  The preconditioner is
+ * M = S (I + L_w) S with s_i = exp(o0_i)/sqrt(N_ii) and L_w the weighted
+ * Laplacian of the 7-point grid graph (w = exp(o)): symmetric positive
+ * definite for any weights, one 7-point stencil per application.  Untrained
+ * (zero output layer), M = D^-1/2 (I + L) D^-1/2.  Training differentiates
+ * unrolled preconditioned CG: stp_lpc_backward_accumulate gives the output
+ * gradient of <g, M r>, stp_lpc_backward takes it through the network and
+ * stp_lpc_adam_step updates the weights.  A handle holds one prepared system
+ * at a time. */
 typedef struct stp_lpc stp_lpc;
 int stp_lpc_create(uint64_t seed, stp_lpc** out);
 int stp_lpc_destroy(stp_lpc* h);
@@ -277,20 +278,24 @@ int stp_lpc_features(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double
                      float* dsq_dev, void* stream);
 /* features + network: prepares M for A (used by stp_cgls precond 2 and stp_lpc_apply) */
 int stp_lpc_prepare(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, void* stream);
-/* coefficients of G for the prepared system, [63][cell]: tap k couples cell
- * i to cell i + d_k, d_k the (62 + k)-th offset of [-2,2]^3 in lexicographic
- * (x, y, z) order (k = 0: the diagonal) */
-int stp_lpc_coefficients(const stp_lpc* h, float* g_dev, void* stream);
+/* the same, keeping the activations and workspace that training needs */
+int stp_lpc_prepare_train(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, void* stream);
+/* coefficients of M for the prepared system, [4][cell]: s, then the weights
+ * of the edges (i, i + e_x), (i, i + e_y), (i, i + e_z) (0 where i + e leaves
+ * the grid) */
+int stp_lpc_coefficients(const stp_lpc* h, float* coef_dev, void* stream);
 /* z = M s for the prepared system */
 int stp_lpc_apply(const stp_lpc* h, const float* s_dev, float* z_dev, void* stream);
-/* log K (without the constant -(1/n) log det J) and its parameter gradient
- * for system A and nprobe probe vectors probes_dev [nprobe][cell] with
- * E[y y^T] = I; grad_dev (optional) [num_params] */
-int stp_lpc_loss_grad(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, const float* probes_dev,
-                      int32_t nprobe, double* loss_host, float* grad_dev, void* stream);
-/* loss_grad followed by one Adam step (beta 0.9 / 0.999) with learning rate lr */
-int stp_lpc_train_step(stp_lpc* h, const stp_csr* A, const int64_t* n_host, double lambda, const float* probes_dev,
-                       int32_t nprobe, double lr, double* loss_host, void* stream);
+/* adds d<g, M r>/d(network output) to the handle's output gradient (cleared
+ * by each prepare and each backward); needs stp_lpc_prepare_train */
+int stp_lpc_backward_accumulate(stp_lpc* h, const float* r_dev, const float* g_dev, void* stream);
+/* scale x the accumulated output gradient -> gradient of the FP32 master
+ * weights, kept in the handle for stp_lpc_adam_step and copied to grad_dev
+ * (optional) [num_params] */
+int stp_lpc_backward(stp_lpc* h, double scale, float* grad_dev, void* stream);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the last backward's gradient;
+ * the prepared system must be prepared again before use */
+int stp_lpc_adam_step(stp_lpc* h, double lr, void* stream);
 /* single convolution layer (zero padding), BF16 input [cell][cin] (raw bits),
  * FP32 weights [cout][27][cin] rounded to BF16, FP32 output [cell][cout];
  * cin, cout multiples of 16 (cout <= 96, != 80) */

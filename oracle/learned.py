@@ -3,18 +3,21 @@
 A float64 torch restatement of what csrc/learned_precond.hip and
 csrc/conv3d.hip compute: the coupling features of N = A^T A + lam I, the
 3-level convolutional network (with the same BF16 roundings of weights and
-activations, or none), the upper-triangular 63-tap factor G, the
-preconditioner M = D^-1/2 G G^T D^-1/2 and the Kaporin loss
-    log K~ = log(q / (P n)) - (2/n) sum_i log G_ii,  q = sum_y (G y)^T J (G y),
-with its gradient by autograd.  The reference has no learned preconditioner
-(its only one is Jacobi, src/cpp/sparse/sparse_ops.cpp:52-113), so this is
-pinned by the checks in tests/test_oracle_learned.py: a brute-force loop
-convolution, the Jacobi limit of an untrained network, finite differences of
-the loss, and the exact Kaporin number on small systems.
+activations, or none), the coefficients s_i = exp(o0_i)/sqrt(N_ii),
+w_e = exp(o_e) of the preconditioner M = S (I + L_w) S (L_w the weighted
+Laplacian of the 7-point grid graph), its application, and the unrolled-CG
+training loss with its gradient by autograd.  The reference has no learned
+preconditioner (its only one is Jacobi, src/cpp/sparse/sparse_ops.cpp:52-113),
+so this is pinned by the checks in tests/test_oracle_learned.py: a
+brute-force loop convolution, an explicit edge-by-edge matrix of M and the
+Kronecker-product grid Laplacian of an untrained network, symmetry and
+positive definiteness, the energy-norm identity of CG, and finite
+differences of the loss.
 
 Layouts follow the library: activations channels-last [cell][channel] with
 cell = (x*n1 + y)*n2 + z; weights [cout][27][cin] with tap t = (dx+1)*9 +
-(dy+1)*3 + (dz+1); coefficients [63][cell].
+(dy+1)*3 + (dz+1); coefficients [4][cell] (s, w_x, w_y, w_z; the weight of
+cell i belongs to the edge (i, i + e_axis) and is 0 at the far face).
 """
 from __future__ import annotations
 
@@ -23,9 +26,11 @@ import scipy.sparse as sp
 import torch
 import torch.nn.functional as F
 
-LAYERS = ((32, 16), (16, 32), (32, 64), (96, 32), (48, 16), (16, 64))
-NG = 63
+LAYERS = ((32, 16), (16, 32), (32, 64), (96, 32), (48, 16), (16, 16))
+NCOEF = 4
 NFEAT = 32
+W0 = 1.0          # edge weight of an untrained network
+OCLAMP = 30.0
 
 
 def bf16(x: torch.Tensor, on: bool = True) -> torch.Tensor:
@@ -41,11 +46,6 @@ def bf16_ste(x: torch.Tensor, on: bool = True) -> torch.Tensor:
 
 def offsets27() -> np.ndarray:
     return np.array([(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)], dtype=np.int64)
-
-
-def g_offsets() -> np.ndarray:
-    o = np.array([(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-2, 3)], dtype=np.int64)
-    return o[62:]
 
 
 def layer_slices():
@@ -159,7 +159,7 @@ def upcat(coarse_cl, fine_cl, grid):
 
 
 def network(params: torch.Tensor, feat: torch.Tensor, grid, round_bf16: bool = True) -> torch.Tensor:
-    """Network output [64][cell] (before the exponential of channel 0)."""
+    """Network output [16][cell] (channels 0..3 used)."""
     r = round_bf16
     W = [bf16_ste(params[a:b], r).reshape(cout, 27, cin) for a, b, cin, cout in layer_slices()]
     g1 = tuple(grid)
@@ -175,79 +175,107 @@ def network(params: torch.Tensor, feat: torch.Tensor, grid, round_bf16: bool = T
     return conv(e, W[5], g1).T.contiguous()
 
 
-def coefficients(out: torch.Tensor) -> torch.Tensor:
-    """[63][cell]: diagonal exp(clamp(out0, -30, 30)), the rest as output."""
-    return torch.cat([torch.exp(out[:1].clamp(-30, 30)), out[1:NG]], dim=0)
-
-
-def apply_g(g: torch.Tensor, y: torch.Tensor, grid) -> torch.Tensor:
-    """(G y)_i = sum_k g[k][i] y[i + d_k] (zero outside the grid)."""
+# ------------------------------------------------------------------ M = S (I + L_w) S
+def coefficients(out: torch.Tensor, dsq: torch.Tensor, grid) -> torch.Tensor:
+    """[4][cell]: s = dsq exp(o0), w_d = W0 exp(o_{1+d}) where cell + e_d is
+    inside the grid, else 0 (outputs clamped to +-30 first)."""
     n0, n1, n2 = grid
-    y3 = F.pad(y.reshape(1, 1, n0, n1, n2), (2, 2, 2, 2, 2, 2))[0, 0]
-    out = torch.zeros_like(y)
-    for k, (a, b, c) in enumerate(g_offsets()):
-        out = out + g[k] * y3[2 + a:2 + a + n0, 2 + b:2 + b + n1, 2 + c:2 + c + n2].reshape(-1)
-    return out
+    o = out[:NCOEF].clamp(-OCLAMP, OCLAMP)
+    x, y, z = np.meshgrid(np.arange(n0), np.arange(n1), np.arange(n2), indexing="ij")
+    inside = [torch.tensor((x + 1 < n0).ravel()), torch.tensor((y + 1 < n1).ravel()), torch.tensor((z + 1 < n2).ravel())]
+    rows = [dsq * torch.exp(o[0])]
+    for d in range(3):
+        rows.append(torch.where(inside[d], W0 * torch.exp(o[1 + d]), torch.zeros_like(o[1 + d])))
+    return torch.stack(rows)
 
 
-def apply_gt(g: torch.Tensor, v: torch.Tensor, grid) -> torch.Tensor:
-    """(G^T v)_j = sum_k g[k][j - d_k] v[j - d_k]."""
+def apply_m(coef: torch.Tensor, r: torch.Tensor, grid) -> torch.Tensor:
+    """z = S (I + L_w) S r: (L_w y)_i = sum over the edges (i, j) of
+    w_ij (y_i - y_j)."""
     n0, n1, n2 = grid
-    out = torch.zeros_like(v)
-    for k, (a, b, c) in enumerate(g_offsets()):
-        w = F.pad((g[k] * v).reshape(1, 1, n0, n1, n2), (2, 2, 2, 2, 2, 2))[0, 0]
-        out = out + w[2 - a:2 - a + n0, 2 - b:2 - b + n1, 2 - c:2 - c + n2].reshape(-1)
-    return out
+    y = (coef[0] * r).reshape(n0, n1, n2)
+    u = y.clone()
+    for d in range(3):
+        w = coef[1 + d].reshape(n0, n1, n2)
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[d] = slice(0, (n0, n1, n2)[d] - 1)
+        hi[d] = slice(1, None)
+        f = w[tuple(lo)] * (y[tuple(lo)] - y[tuple(hi)])
+        pad_hi = [0] * 6
+        pad_hi[2 * (2 - d) + 1] = 1
+        pad_lo = [0] * 6
+        pad_lo[2 * (2 - d)] = 1
+        u = u + F.pad(f, pad_hi) - F.pad(f, pad_lo)
+    return coef[0] * u.reshape(-1)
 
 
-def dense_g(g: torch.Tensor, grid) -> torch.Tensor:
-    """G as a dense matrix (tiny grids)."""
-    V = int(np.prod(grid))
-    return torch.stack([apply_g(g, e, grid) for e in torch.eye(V, dtype=g.dtype)], dim=1)
+def dense_m(coef: torch.Tensor, grid) -> np.ndarray:
+    """M as a dense matrix, assembled edge by edge (tiny grids)."""
+    n0, n1, n2 = grid
+    V = n0 * n1 * n2
+    c = coef.detach().numpy()
+    P = np.eye(V)
+    st = (n1 * n2, n2, 1)
+    for i in range(V):
+        for d in range(3):
+            w = c[1 + d, i]
+            if w == 0.0:
+                continue
+            j = i + st[d]
+            P[i, i] += w
+            P[j, j] += w
+            P[i, j] -= w
+            P[j, i] -= w
+    return c[0][:, None] * P * c[0][None, :]
 
 
-def apply_m(g, dsq, s, grid):
-    """z = D^-1/2 G G^T D^-1/2 s."""
-    return dsq * apply_g(g, apply_gt(g, dsq * s, grid), grid)
+def pair_output_gradient(out: torch.Tensor, dsq: torch.Tensor, grid, r, g) -> torch.Tensor:
+    """d<g, M r>/d(network output) [16][cell] (what stp_lpc_backward_accumulate
+    adds up, through the exponentials)."""
+    o = out.detach().clone().requires_grad_(True)
+    val = torch.as_tensor(g, dtype=torch.float64) @ apply_m(coefficients(o, dsq, grid),
+                                                           torch.as_tensor(r, dtype=torch.float64), grid)
+    val.backward()
+    return o.grad
 
 
-def kaporin_loss(g: torch.Tensor, J, probes: torch.Tensor, grid, diag_exponent: torch.Tensor | None = None):
-    """log(q / (P n)) - (2/n) sum log G_ii with q = sum_y (G y)^T J (G y);
-    J is a callable x -> J x.  diag_exponent (the network output of channel
-    0, clamped) replaces log G_ii when given."""
-    n = g.shape[1]
-    q = 0.0
-    for y in probes:
-        u = apply_g(g, y, grid)
-        q = q + u @ J(u)
-    logd = diag_exponent.sum() if diag_exponent is not None else torch.log(g[0]).sum()
-    return torch.log(q / (probes.shape[0] * n)) - 2.0 * logd / n
+def unrolled_cg_loss(coef: torch.Tensor, N: torch.Tensor, rhs: torch.Tensor, x_star: torch.Tensor, grid,
+                     k1: int, k2: int) -> torch.Tensor:
+    """Preconditioned CG from x = 0 for k2 iterations; the mean over k in
+    [k1, k2] of 0.5 log(e_k . r_k / x_star . rhs), e_k = x_star - x_k."""
+    x = torch.zeros_like(rhs)
+    r = rhs.clone()
+    e0 = x_star @ rhs
+    z = apply_m(coef, r, grid)
+    p = z
+    rz = r @ z
+    terms = []
+    for k in range(k2):
+        q = N @ p
+        a = rz / (p @ q)
+        x = x + a * p
+        r = r - a * q
+        if k + 1 >= k1:
+            terms.append(0.5 * torch.log(((x_star - x) @ r) / e0))
+        if k + 1 == k2:
+            break
+        z = apply_m(coef, r, grid)
+        rz2 = r @ z
+        p = z + (rz2 / rz) * p
+        rz = rz2
+    return torch.stack(terms).mean()
 
 
-def kaporin_exact(g: torch.Tensor, Jd: torch.Tensor, grid) -> float:
-    """Exact log K(G^T J G) = log(tr(B)/n) - (1/n) log det B (dense, tiny grids)."""
-    Gd = dense_g(g, grid)
-    B = Gd.T @ Jd @ Gd
-    n = B.shape[0]
-    return float(torch.log(torch.trace(B) / n) - torch.logdet(B) / n)
-
-
-def j_operator(N: sp.csr_matrix, dsq: torch.Tensor):
-    Nt = torch.tensor(N.toarray())
-
-    def J(x):
-        return dsq * (Nt @ (dsq * x))
-    return J
-
-
-def loss_and_grad(params_np: np.ndarray, N: sp.csr_matrix, grid, lam: float, probes: np.ndarray,
+def loss_and_grad(params_np: np.ndarray, N: sp.csr_matrix, grid, lam: float, rhs, x_star, k1: int, k2: int,
                   round_bf16: bool = True):
-    """Oracle of stp_lpc_loss_grad: (loss, gradient w.r.t. the FP32 master weights)."""
+    """Oracle of one training step: (unrolled-CG loss, gradient with respect
+    to the FP32 master weights, coefficients, dsq)."""
     feat, dsq = features(N, grid, lam, round_bf16)
     p = torch.tensor(np.asarray(params_np, np.float64), requires_grad=True)
     out = network(p, feat, grid, round_bf16)
-    g = coefficients(out)
-    loss = kaporin_loss(g, j_operator(N, dsq), torch.tensor(np.asarray(probes, np.float64)), grid,
-                        diag_exponent=out[0].clamp(-30, 30))
+    coef = coefficients(out, dsq, grid)
+    loss = unrolled_cg_loss(coef, torch.tensor(N.toarray()), torch.as_tensor(rhs, dtype=torch.float64),
+                            torch.as_tensor(x_star, dtype=torch.float64), grid, k1, k2)
     loss.backward()
-    return float(loss.detach()), p.grad.numpy().copy(), g.detach(), dsq
+    return float(loss.detach()), p.grad.numpy().copy(), coef.detach(), dsq
