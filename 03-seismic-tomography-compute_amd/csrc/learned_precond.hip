@@ -84,31 +84,65 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
 }
 
 // ------------------------------------------------------------------ features
-// One wavefront per row of A (a ray).  Columns of a row are sorted, so the
-// neighbours of cell i in the row follow it within n1*n2 + n2 + 1 of its index.
-__global__ void __launch_bounds__(256) couplings_kernel(int rows, const int* __restrict__ rp,
-                                                        const int* __restrict__ ci, const float* __restrict__ val,
-                                                        int n0, int n1, int n2, double* nloc) {
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= rows) return;
-    const int lane = threadIdx.x & 63;
-    const int beg = rp[row], end = rp[row + 1];
+// N_ij = sum_r a_ri a_rj for each cell i and its 26 grid neighbours j, from
+// the rows of A^T (cell -> sorted ray ids and lengths): N_ij is the sparse
+// dot product of rows i and j.  One wavefront per cell i (four per block):
+// the ray ids of row i are staged in LDS (rows longer than ROWCAP are
+// searched in global memory instead); the diagonal is the lane-strided sum of
+// squares of row i; for each of the 13 forward neighbours j = i + d the lanes
+// take the entries of row j and look each ray up in row i by binary search,
+// then the wavefront sums the products.  The value goes to channel s of i and
+// channel 26 - s of j: each location has exactly one writer, no atomics, and
+// the summation order is fixed.
+constexpr int ROWCAP = 1024;
+
+__device__ __forceinline__ int lower_bound_i32(const int* a, int n, int key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (a[m] < key) lo = m + 1; else hi = m;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) couplings_kernel(int n0, int n1, int n2, const int* __restrict__ trp,
+                                                        const int* __restrict__ tci,
+                                                        const float* __restrict__ tval, double* nloc) {
+    __shared__ int rowi[4][ROWCAP];
     const int64_t V = (int64_t)n0 * n1 * n2;
-    const int reach = n1 * n2 + n2 + 1;
-    for (int k = beg + lane; k < end; k += 64) {
-        const int i = ci[k];
-        const double a = val[k];
-        const int zi = i % n2, yi = (i / n2) % n1, xi = i / (n1 * n2);
-        unsafeAtomicAdd(nloc + 13 * V + i, a * a);
-        for (int k2 = k + 1; k2 < end; ++k2) {
-            const int j = ci[k2];
-            if (j - i > reach) break;
-            const int dz = j % n2 - zi, dy = (j / n2) % n1 - yi, dx = j / (n1 * n2) - xi;
-            if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) continue;
-            const int s = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-            const double pr = a * (double)val[k2];
-            unsafeAtomicAdd(nloc + s * V + i, pr);
-            unsafeAtomicAdd(nloc + (26 - s) * V + j, pr);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+    const bool live = i < V;
+    const int a0 = live ? trp[i] : 0, a1 = live ? trp[i + 1] : 0;
+    const int na = a1 - a0;
+    const bool staged = na <= ROWCAP;
+    if (staged)
+        for (int k = lane; k < na; k += 64) rowi[wv][k] = tci[a0 + k];
+    __syncthreads();
+    if (!live) return;
+    const int* ri = staged ? rowi[wv] : tci + a0;
+    double d = 0.0;
+    for (int k = a0 + lane; k < a1; k += 64) d += (double)tval[k] * (double)tval[k];
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) d += __shfl_xor(d, m);
+    if (lane == 0) nloc[13 * V + i] = d;
+    const int z = (int)(i % n2), y = (int)((i / n2) % n1), x = (int)(i / ((int64_t)n1 * n2));
+    for (int sidx = 14; sidx < 27; ++sidx) {
+        const int dx = sidx / 9 - 1, dy = (sidx / 3) % 3 - 1, dz = sidx % 3 - 1;
+        if (x + dx >= n0 || y + dy < 0 || y + dy >= n1 || z + dz < 0 || z + dz >= n2) continue;
+        const int64_t j = ((int64_t)(x + dx) * n1 + (y + dy)) * n2 + (z + dz);
+        const int b0 = trp[j], b1 = trp[j + 1];
+        double s = 0.0;
+        for (int k = b0 + lane; k < b1; k += 64) {
+            const int rb = tci[k];
+            const int lo = lower_bound_i32(ri, na, rb);
+            if (lo < na && ri[lo] == rb) s += (double)tval[a0 + lo] * (double)tval[k];
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m);
+        if (lane == 0) {
+            nloc[(int64_t)sidx * V + i] = s;
+            nloc[(int64_t)(26 - sidx) * V + j] = s;
         }
     }
 }
@@ -398,9 +432,9 @@ int check_system(const stp_csr* A, const int64_t* n) {
 int features(stp_lpc* h, const stp_csr* A, double lam, float* feat32, hipStream_t st) {
     const int64_t V = h->V;
     STP_HIP(hipMemsetAsync(h->nloc, 0, sizeof(double) * V * NTAP, st));
-    if (A->rows > 0)
-        hipLaunchKernelGGL(couplings_kernel, dim3((A->rows + 3) / 4), dim3(256), 0, st, A->rows, A->rp, A->ci, A->val,
-                           h->n0, h->n1, h->n2, h->nloc);
+    if (V > 0)
+        hipLaunchKernelGGL(couplings_kernel, dim3((unsigned)((V + 3) / 4)), dim3(256), 0, st, h->n0, h->n1, h->n2,
+                           A->trp, A->tci, A->tval, h->nloc);
     STP_HIP(hipGetLastError());
     hipLaunchKernelGGL(features_kernel, dim3(gridn(V)), dim3(256), 0, st, h->n0, h->n1, h->n2, h->nloc, lam, h->feat,
                        h->dsq, feat32);

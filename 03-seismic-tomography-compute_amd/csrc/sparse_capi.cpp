@@ -50,8 +50,8 @@ static int finish_csr(stp_csr* h, hipStream_t st) {
     STP_HIP(hipMalloc(&h->tci, sizeof(int) * (h->nnz ? h->nnz : 1)));
     STP_HIP(hipMalloc(&h->tval, sizeof(float) * (h->nnz ? h->nnz : 1)));
     STP_HIP(stp::csr_transpose(h->rows, h->cols, h->nnz, h->rp, h->ci, h->val, h->trp, h->tci, h->tval, st));
-    h->W = stp::spmv_lanes(h->rows ? (double)h->nnz / h->rows : 0.0);
-    h->WT = stp::spmv_lanes(h->cols ? (double)h->nnz / h->cols : 0.0);
+    h->W = stp::spmv_lanes(h->rows ? (double)h->nnz / h->rows : 0.0, 0);
+    h->WT = stp::spmv_lanes(h->cols ? (double)h->nnz / h->cols : 0.0, 1);
     return STP_OK;
 }
 

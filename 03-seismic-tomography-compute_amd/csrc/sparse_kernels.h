@@ -11,7 +11,7 @@ struct SolverScalars {
     int iters;
 };
 
-int spmv_lanes(double mean_nnz);
+int spmv_lanes(double mean_nnz, int transpose);
 int spmv_blocks(int rows, int W);
 hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
                     double* part, const int* done, hipStream_t st);

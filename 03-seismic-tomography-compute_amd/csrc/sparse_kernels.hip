@@ -27,22 +27,38 @@ namespace stp {
 // and each block leaves one FP64 partial per reduced quantity.
 constexpr int MAX_BLOCKS = 2048;
 
+// Each lane takes entries b+lane, b+lane+W, ... and accumulates them in that
+// order.  The row is consumed in chunks of U*W entries (U*W = 128): all U
+// value/column loads of a chunk are issued, then all U gathers of x, then
+// the U products are added, so a lane has U independent gathers in flight
+// instead of a chain of dependent load pairs.  Slots past the row end load
+// the row's first entry and contribute an exact zero, which leaves the sum
+// (and its order) unchanged.
 template <int W, typename AT, typename XT>
 __device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const int* __restrict__ ci,
                                       const float* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
+    constexpr int U = (128 / W) < 1 ? 1 : (128 / W);
     AT acc = 0;
     const int b = rp[row], e = rp[row + 1];
-    int k = b + lane;
-    for (; k + 3 * W < e; k += 4 * W) {
-        const float v0 = val[k], v1 = val[k + W], v2 = val[k + 2 * W], v3 = val[k + 3 * W];
-        const int c0 = ci[k], c1 = ci[k + W], c2 = ci[k + 2 * W], c3 = ci[k + 3 * W];
-        const AT x0 = (AT)x[c0], x1 = (AT)x[c1], x2 = (AT)x[c2], x3 = (AT)x[c3];
-        acc += (AT)v0 * x0;
-        acc += (AT)v1 * x1;
-        acc += (AT)v2 * x2;
-        acc += (AT)v3 * x3;
+    for (int k0 = b + lane; k0 < e; k0 += U * W) {
+        float v[U];
+        int c[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + u * W;
+            ok[u] = k < e;
+            const int kk = ok[u] ? k : b;
+            v[u] = val[kk];
+            c[u] = ci[kk];
+        }
+        AT xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = (AT)x[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (ok[u]) acc += (AT)v[u] * xv[u];
     }
-    for (; k < e; k += W) acc += (AT)val[k] * (AT)x[ci[k]];
 #pragma unroll
     for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
     return acc;
@@ -94,11 +110,17 @@ static hipError_t spmv_w(int rows, const int* rp, const int* ci, const float* va
     return hipGetLastError();
 }
 
-int spmv_lanes(double mean_nnz) {
-    if (const char* ev = getenv("STP_SPMV_W")) {
+// Lanes per row.  A (rays x cells, FP32 gathers of a cell vector) and A^T
+// (cells x rays, FP64 gathers of a ray vector) are chosen separately: at
+// 200k rays x 64^3 cells (107 and 81 entries per row) A runs fastest with
+// 32 lanes and A^T with 64 (a whole wavefront gathers the rays of one cell,
+// which are close in ray order).  STP_SPMV_W / STP_SPMVT_W override.
+int spmv_lanes(double mean_nnz, int transpose) {
+    if (const char* ev = getenv(transpose ? "STP_SPMVT_W" : "STP_SPMV_W")) {
         const int w = atoi(ev);
         if (w == 8 || w == 16 || w == 32 || w == 64) return w;
     }
+    if (transpose && mean_nnz >= 48) return 64;
     return mean_nnz >= 96 ? 32 : (mean_nnz >= 24 ? 16 : 8);
 }
 
@@ -334,12 +356,21 @@ hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* 
 }
 
 // Jacobi for A^T A + lam I: dinv_j = 1 / (sum_i a_ij^2 + lam), from the rows of A^T
-__global__ void normal_diag_kernel(int n, const int* rp, const float* val, double lam, float* dinv) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    double s = lam;
-    for (int k = rp[j]; k < rp[j + 1]; ++k) s += (double)val[k] * (double)val[k];
-    dinv[j] = s > 0.0 ? (float)(1.0 / s) : 1.0f;
+// (16 lanes per row of A^T, lane-strided sums combined by shuffles)
+__global__ void __launch_bounds__(256) normal_diag_kernel(int n, const int* rp, const float* val, double lam,
+                                                           float* dinv) {
+    constexpr int W = 16;
+    const int lane = threadIdx.x % W;
+    const int j = blockIdx.x * (256 / W) + (int)threadIdx.x / W;
+    double s = 0.0;
+    if (j < n)
+        for (int k = rp[j] + lane; k < rp[j + 1]; k += W) s += (double)val[k] * (double)val[k];
+#pragma unroll
+    for (int m = W / 2; m > 0; m >>= 1) s += __shfl_xor(s, m);
+    if (j < n && lane == 0) {
+        s += lam;
+        dinv[j] = s > 0.0 ? (float)(1.0 / s) : 1.0f;
+    }
 }
 
 // Jacobi for a square A (reference jacobi_preconditioned_cg): first diagonal
@@ -677,7 +708,8 @@ hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st) {
     return hipGetLastError();
 }
 hipError_t launch_normal_diag(int n, const int* rp, const float* val, double lam, float* dinv, hipStream_t st) {
-    hipLaunchKernelGGL(normal_diag_kernel, dim3(nb256(n)), dim3(256), 0, st, n, rp, val, lam, dinv);
+    hipLaunchKernelGGL(normal_diag_kernel, dim3((unsigned)((n + 15) / 16 > 0 ? (n + 15) / 16 : 1)), dim3(256), 0, st,
+                       n, rp, val, lam, dinv);
     return hipGetLastError();
 }
 hipError_t launch_diag_inv(int n, const int* rp, const int* ci, const float* val, double* dinv, hipStream_t st) {

@@ -327,14 +327,17 @@ TRAIN_MIX = (((32, 32, 32), 50_000), ((48, 48, 48), 112_000), ((64, 64, 64), 200
 VALIDATION = (((64, 64, 64), 200_000, 90_000), ((48, 48, 48), 112_000, 90_002))
 
 
-def train_recipe(seed: int = 0, n_systems: int = 120, val_every: int = 20, k_window=(20, 100), lr: float = 1e-3,
-                 lr_final: float = 1e-4, log_every: int = 0):
+def train_recipe(seed: int = 0, n_systems: int = 120, val_every: int = 20, k_window=(20, 100), lr: float = 1e-4,
+                 lr_final: float = 3e-5, log_every: int = 0):
     """The training run behind the default weights: Adam on the unrolled-CG
     loss over `n_systems` seeded systems cycling through TRAIN_MIX (seeds
     from 10_000 on, disjoint from the validation seeds and from the
     benchmark's), one step per system; every `val_every` systems the
     VALIDATION systems are solved with learned CGLS and the weights with the
-    fewest iterations in total are kept (the untrained network included)."""
+    fewest iterations in total are kept (the untrained network included).
+    The rate is small on purpose: measured on 32^3 held-out systems, Adam at
+    1e-3 on this loss raises the iteration count of the untrained network
+    (66 -> 159), at 1e-4 it stays within a few iterations of it."""
     net = LearnedPreconditioner(seed=seed)
     lam = 1e-2
     val = [make_system(g, r, sd, lam) for g, r, sd in VALIDATION]
@@ -373,7 +376,7 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     ap.add_argument("--systems", type=int, default=120)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=1e-4)
     args = ap.parse_args()
     net, info = train_recipe(seed=args.seed, n_systems=args.systems, lr=args.lr, log_every=10)
     __import__("pathlib").Path(args.out).parent.mkdir(parents=True, exist_ok=True)

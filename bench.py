@@ -85,7 +85,8 @@ def accuracy_check(dev, threads: int, steps: int = 500):
     return {"case": f"128^3 two-layer, {steps} steps", "field_rel_l2_vs_fp64": e, "traces_rel_l2_vs_fp64": et}
 
 
-def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: float = 1e-6, reps: int = 3):
+def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: float = 1e-6, reps: int = 3,
+                kinds=("jacobi", "none", "learned")):
     """Tomography solve: exact straight-ray CSR (seeded source/receiver pairs
     on opposite faces of a 64^3 model), b = A s_true, CGLS on
     (A^T A + lam I) x = A^T b to a 1e-6 relative normal-equation residual."""
@@ -115,7 +116,9 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
     out["spmv"] = {"us": round(us, 2), "gb_s": round(spmv_bytes / (us * 1e-6) / 1e9, 1),
                    "frac_hbm": round(spmv_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
     from seismic_engine.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, train_recipe
-    if DEFAULT_WEIGHTS.exists():
+    if "learned" not in kinds:
+        net = None
+    elif DEFAULT_WEIGHTS.exists():
         net = LearnedPreconditioner.load(DEFAULT_WEIGHTS)
         out["learned_weights"] = str(DEFAULT_WEIGHTS.relative_to(ROOT))
     else:
@@ -124,7 +127,7 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
         net, tinfo = train_recipe()
         out["learned_weights"] = f"trained in this run ({time.perf_counter() - t0:.1f} s)"
     grid = (n, n, n)
-    for kind in ("jacobi", "none", "learned"):
+    for kind in kinds:
         best = None
         for _ in range(reps):
             torch.cuda.synchronize()
@@ -143,7 +146,8 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
                      "true_rel_residual": info.true_rel_residual, "replacements": info.replacements}
         if kind == "learned":
             out[kind]["prepare_seconds"] = round(tp, 5)
-    out["learned_over_jacobi_time"] = round(out["learned"]["seconds"] / out["jacobi"]["seconds"], 4)
+    if "learned" in out and "jacobi" in out:
+        out["learned_over_jacobi_time"] = round(out["learned"]["seconds"] / out["jacobi"]["seconds"], 4)
     return out
 
 
@@ -157,6 +161,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cg", action="store_true", help="skip the tomography-solve timing")
+    ap.add_argument("--cg-only", action="store_true",
+                    help="only the tomography-solve timing (one JSON object, for profiling the CGLS kernels)")
+    ap.add_argument("--cg-kinds", default="jacobi,none,learned",
+                    help="comma-separated CGLS preconditioners to time (jacobi, none, learned)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for PMC byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
@@ -179,6 +187,9 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 4 << 28, "bytes_written": 4 << 28, "copies": 3}))
+        return 0
+    if args.cg_only:
+        print(json.dumps({"cg": cg_workload(dev, kinds=tuple(args.cg_kinds.split(",")))}), flush=True)
         return 0
     n, nt, hw = args.n, args.nt, 4
     log(f"building {n}^3 velocity (seed 2026, sigma 20) on the GPU")
