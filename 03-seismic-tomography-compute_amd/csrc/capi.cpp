@@ -54,6 +54,7 @@ void fd_fill_args(const stp_fd_plan* p, FdStepArgs* a) {
     a->sw = c.sponge_width;
     a->sigma_over_sw2 = c.sponge_width > 0 ? (float)(c.sponge_damping / ((double)c.sponge_width * c.sponge_width)) : 0.f;
     a->amp = (float)c.amplitude_limit;
+    a->damp_next = p->sponge_mode == STP_SPONGE_CERJAN;
     a->ntz = p->ntz; a->nty = p->nty; a->nxc = p->nxc; a->nshots = c.nshots;
     a->nrcv = p->nrcv;
     a->blk_rcv_ptr = p->d_blk_ptr;
@@ -270,6 +271,14 @@ int stp_fd_coefficients(stp_fd_plan* p, const float* vel, float* coef, double* v
     return STP_OK;
 }
 
+int stp_fd_set_sponge_mode(stp_fd_plan* p, int32_t mode) {
+    if (!p) return fail(STP_ERR_INVALID, "stp_fd_set_sponge_mode: null plan");
+    if (mode != STP_SPONGE_REFERENCE && mode != STP_SPONGE_CERJAN)
+        return fail(STP_ERR_INVALID, "stp_fd_set_sponge_mode: unknown mode");
+    p->sponge_mode = mode;
+    return STP_OK;
+}
+
 int stp_fd_verify_cfl(const stp_fd_plan* p, double vmax, int32_t* ok) {
     if (!p || !ok) return fail(STP_ERR_INVALID, "stp_fd_verify_cfl: null argument");
     const stp_fd_config& c = p->cfg;
@@ -317,7 +326,8 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
     // odd pair count runs its first pair as two single (in-place) steps; an
     // odd step count ends with one single step, which swaps the roles as the
     // single-step path does.
-    int npairs = p->pair_ok ? nsteps / 2 : 0;
+    // (the two-step kernel implements the reference sponge only)
+    int npairs = p->pair_ok && p->sponge_mode == STP_SPONGE_REFERENCE ? nsteps / 2 : 0;
     const int nsingle_first = (npairs % 2) ? 2 : 0;
     if (npairs % 2) --npairs;
     if (npairs > 0 && !p->d_pair) {

@@ -19,6 +19,7 @@ struct FdStepArgs {
     float sigma_over_sw2;     // sponge_damping / sw^2
     float amp;                // amplitude clamp
     int damp_prev;            // apply the sponge factor to the previous field
+    int damp_next;            // Cerjan sponge: the new field is multiplied by D too
     int ntz, nty, nxc, nshots;  // tiles along z, y; x chunks; shots
     // receivers (per-tile CSR lists)
     int nrcv_pts;             // total list entries (0 = no receivers)
@@ -41,7 +42,7 @@ struct FdStepArgs {
     // mode 2 (backward pass): acc_out += N * acc_src, acc_ill += acc_src^2
     // (interior-indexed, per shot), then the receivers inject
     // inj_traces[(shot*nrcv + r)*inj_stride + inj_col] (times C at the
-    // receiver when inj_scale_c) into N
+    // receiver when inj_scale_c, times D there in Cerjan mode) into N
     const float* acc_src;
     float* acc_out;
     float* acc_ill;

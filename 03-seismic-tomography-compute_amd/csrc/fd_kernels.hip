@@ -25,6 +25,10 @@
 //     (neighbouring tiles share that XCD's L2 for their halos).
 // Sponge: D = min(Dx(plane), Dyz(cell)) because the reference factor is a
 // monotone function of min(distance) — Dyz is computed once per thread.
+// Cerjan mode (damp_next, not in the reference) multiplies the result by D as
+// well, N = D (U + (U - D*P) + C*L(U)): the reference form damps only the
+// previous field, which makes smooth modes inside the sponge grow (root
+// 1 + sqrt(1 - D) of its recurrence); this one decays (root D).
 // Finite guards run only when the result is non-finite (a non-finite input
 // always makes it so), which reproduces the reference's eager checks.
 #include <hip/hip_runtime.h>
@@ -227,6 +231,7 @@ fd_step_kernel(const FdStepArgs a) {
 
     const float amp = a.amp;
     const bool damp_prev = a.damp_prev != 0;
+    const bool damp_next = a.damp_next != 0;
     bool guard_hit = false;
 
     int it = 0;
@@ -304,9 +309,11 @@ fd_step_kernel(const FdStepArgs a) {
                     lap = fmaf(a.cz[k], zs, lap);
                 }
                 const float p = V::get(pc[r], j);
-                const float d = damp_prev ? fminf(dxs, dyz[r][j]) : 1.0f;
+                const float ds = fminf(dxs, dyz[r][j]);
+                const float d = damp_prev ? ds : 1.0f;
                 const float c = V::get(cc[r], j);
-                const float nv = u + (fmaf(-d, p, u) + c * lap);
+                float nv = u + (fmaf(-d, p, u) + c * lap);
+                if (damp_next) nv *= ds;
                 if constexpr (MODE == FD_MODE_SNAPSHOT) V::set(lapv[r], j, lap);
                 bad |= !finitef(nv) && rvalid[r];
                 V::set(res[r], j, fminf(fmaxf(nv, -amp), amp));
@@ -325,7 +332,8 @@ fd_step_kernel(const FdStepArgs a) {
                     const float u = V::get(q[R][r], j);
                     const float p = V::get(pc[r], j);
                     const float c = V::get(cc[r], j);
-                    const float d = damp_prev ? fminf(dxs, dyz[r][j]) : 1.0f;
+                    const float ds = fminf(dxs, dyz[r][j]);
+                    const float d = damp_prev ? ds : 1.0f;
                     if (!rvalid[r]) continue;
                     const uint32_t ci = pb + roff[r] + j;
                     bool ok_n = true;
@@ -340,12 +348,14 @@ fd_step_kernel(const FdStepArgs a) {
                         lap = fmaf(a.cy[k], (yp - u) + (ym - u), lap);
                         lap = fmaf(a.cz[k], (zp - u) + (zm - u), lap);
                     }
+                    // Cerjan mode multiplies the reference result by D
+                    const float dn = damp_next ? ds : 1.0f;
                     float nv;
                     if (!(finitef(u) && finitef(p))) nv = 0.f;
-                    else if (!ok_n) nv = u;
+                    else if (!ok_n) nv = dn * u;
                     else {
-                        nv = u + (fmaf(-d, p, u) + c * lap);
-                        nv = fminf(fmaxf(finitef(nv) ? nv : u, -amp), amp);
+                        nv = dn * (u + (fmaf(-d, p, u) + c * lap));
+                        nv = fminf(fmaxf(finitef(nv) ? nv : dn * u, -amp), amp);
                     }
                     V::set(res[r], j, nv);
                 }
@@ -405,9 +415,13 @@ fd_step_kernel(const FdStepArgs a) {
                 const int rid = a.rcv_id[p];
                 float amt = a.inj_traces[((int64_t)shot * a.nrcv + rid) * a.inj_stride + a.inj_col];
                 if (a.inj_scale_c) amt *= C[ci];
+                const int cx = (int)(ci / (int64_t)plane), cy = (int)((ci / nzh) % nyh), cz = (int)(ci % nzh);
+                if (damp_next)
+                    amt *= sponge_of(min(sponge_dist(cx, nxh, hw, sw), min(sponge_dist(cy, nyh, hw, sw),
+                                                                           sponge_dist(cz, nzh, hw, sw))),
+                                     sw, sg);
                 if (!finitef(amt)) continue;
                 atomicAdd(&P[ci], amt);
-                const int cx = (int)(ci / (int64_t)plane), cy = (int)((ci / nzh) % nyh), cz = (int)(ci % nzh);
                 const int64_t ii = (int64_t)shot * a.nint + ((int64_t)(cx - hw) * a.ny + (cy - hw)) * a.nz + (cz - hw);
                 atomicAdd(&a.acc_out[ii], amt * S[ci]);
             }

@@ -11,6 +11,7 @@ struct stp_fd_plan {
     int nxh = 0, nyh = 0, nzh = 0;
     int64_t ncell = 0;
     float cx[5] = {0}, cy[5] = {0}, cz[5] = {0};
+    int sponge_mode = STP_SPONGE_REFERENCE;   // stp_fd_set_sponge_mode
     // receivers
     int nrcv = 0, nrcv_pts = 0;
     int* d_blk_ptr = nullptr;

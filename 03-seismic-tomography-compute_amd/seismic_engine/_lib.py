@@ -56,6 +56,7 @@ SIGNATURES: dict[str, list] = {
     "stp_fd_plan_destroy": [_vp],
     "stp_fd_set_receivers": [_vp, _vp, _i32],
     "stp_fd_set_sources": [_vp, _vp, _i32],
+    "stp_fd_set_sponge_mode": [_vp, _i32],
     "stp_fd_coefficients": [_vp, _vp, _vp, C.POINTER(_f64), _vp],
     "stp_fd_verify_cfl": [_vp, _f64, C.POINTER(_i32)],
     "stp_fd_propagate": [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp],

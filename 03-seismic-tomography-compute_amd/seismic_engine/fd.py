@@ -21,6 +21,9 @@ def _torch():
     return torch
 
 
+SPONGE_MODES = {"reference": 0, "cerjan": 1}
+
+
 class FDPropagator:
     """Leapfrog propagator for `nshots` wavefields sharing one velocity model.
 
@@ -28,12 +31,14 @@ class FDPropagator:
     order 8 is the radius-4 star stencil.  Both advance
         next = 2*cur - D*prev_raw + v^2 dt^2 * Lap(cur)
     where D is the reference sponge factor (wave_equation.cpp:262-297).
+    sponge="cerjan" multiplies `next` by D as well (not in the reference): the
+    reference form grows inside the sponge on long runs, this one decays.
     """
 
     def __init__(self, nx: int, ny: int, nz: int, dx: float = 10.0, dy: float = 10.0, dz: float = 10.0,
                  dt: float = 1e-3, order: int = 8, halo_width: int | None = None, sponge_width: int = 20,
                  sponge_damping: float = 0.015, amplitude_limit: float = 1e10, cfl_safety_factor: float = 0.49,
-                 nshots: int = 1, device: str = "cuda"):
+                 nshots: int = 1, device: str = "cuda", sponge: str = "reference"):
         torch = _torch()
         if halo_width is None:
             halo_width = order // 2
@@ -51,6 +56,10 @@ class FDPropagator:
         h = C.c_void_p()
         call("stp_fd_plan_create", C.byref(self.cfg), C.byref(h))
         self._plan = h
+        if sponge not in SPONGE_MODES:
+            raise ValueError(f"sponge must be one of {sorted(SPONGE_MODES)}")
+        self.sponge = sponge
+        call("stp_fd_set_sponge_mode", self._plan, SPONGE_MODES[sponge])
         shape = (self.nshots,) + self.nh
         self.cur = torch.zeros(shape, dtype=torch.float32, device=self.device)
         self.prev = torch.zeros(shape, dtype=torch.float32, device=self.device)

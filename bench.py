@@ -148,6 +148,90 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
             out[kind]["prepare_seconds"] = round(tp, 5)
     if "learned" in out and "jacobi" in out:
         out["learned_over_jacobi_time"] = round(out["learned"]["seconds"] / out["jacobi"]["seconds"], 4)
+    if net is not None:
+        log("learned preconditioner on the held-out 96^3 system")
+        out["heldout_96"] = learned_heldout(dev, net=net)
+    return out
+
+
+def rtm_workload(fd, nt: int, w, traces):
+    """RTM imaging on the benchmark model: the source wavefield is propagated
+    forward with sqrt(nt) checkpoints, the recorded traces are injected in
+    reverse time and the two wavefields are cross-correlated at zero lag
+    (image and illumination) inside the reverse step kernel."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fd.reverse_time(nt, w, inject=traces, snapshot="field", scale_by_c=False, illumination=True, record=False)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    img, ill = out["acc"], out["ill"]
+    ok = bool(torch.isfinite(img).all()) and bool((ill >= 0).all()) and float(ill.max()) > 0
+    n3 = img[0].numel()
+    del out, img, ill
+    torch.cuda.empty_cache()
+    return {"seconds": round(sec, 4), "image_gcell_steps_per_s": round(n3 * nt / sec / 1e9, 2), "finite": ok,
+            "passes": "forward with checkpoints, segment recompute with snapshots, reverse step with fused "
+                      "image/illumination accumulation"}
+
+
+def gradient_workload(dev, n: int = 256, nshots: int = 16, nt: int = 1500):
+    """Adjoint-state gradient: n^3 grid, 16 shots on a 4 x 4 surface grid
+    (iz = 2), receivers every 4th surface cell, smooth random true and
+    starting velocities (seeds 1 and 2, sigma 20 cells, 1500-4500 m/s), L2
+    waveform misfit; observed data from the true model, then one misfit +
+    gradient evaluation (forward, checkpointed, and adjoint) is timed."""
+    import numpy as np
+    import torch
+    from seismic_engine import FDPropagator, synthetic
+    hw = 4
+    w = torch.as_tensor(synthetic.ricker(15.0, 1e-3, nt), dtype=torch.float32, device=dev)
+    pos = [n // 8 * (1 + 2 * k) for k in range(4)]
+    rcv = synthetic.surface_receivers(n, n, n, hw)[::1].reshape(n, n)[::4, ::4].reshape(-1)
+    fd = FDPropagator(n, n, n, order=8, sponge_width=20, sponge_damping=0.015, nshots=nshots, device=dev)
+    fd.set_sources([fd.index(ix, iy, 2) for ix in pos for iy in pos])
+    fd.set_receivers(rcv)
+    fd.set_velocity(synthetic.smooth_random_velocity(n, n, n, seed=1, sigma=20.0, device=dev, as_tensor=True))
+    obs = fd.propagate(nt, w).clone()
+    fd.reset()
+    fd.set_velocity(synthetic.smooth_random_velocity(n, n, n, seed=2, sigma=20.0, device=dev, as_tensor=True))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, g, _ = fd.misfit_gradient(nt, w, obs)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    ok = bool(torch.isfinite(g).all()) and np.isfinite(J) and float(g.abs().max()) > 0
+    cells = float(n) ** 3 * nshots * nt
+    del fd, obs, g
+    torch.cuda.empty_cache()
+    return {"grid": [n, n, n], "shots": nshots, "time_steps": nt, "receivers": int(rcv.size), "seconds": round(sec, 3),
+            "shot_gcell_steps_per_s": round(cells / sec / 1e9, 2), "misfit": J, "finite": ok}
+
+
+def learned_heldout(dev, n: int = 96, lam: float = 1e-2, tol: float = 1e-6, net=None):
+    """Held-out larger system of the learned-preconditioner recipe: n^3 cells,
+    rays at the 200k-ray 64^3 density per face cell (seed 10_096), Jacobi
+    against learned CGLS to 1e-6 (learned time includes its preparation)."""
+    import torch
+    from seismic_engine.learned import make_system
+    nrays = int(200_000 * (n / 64) ** 2)
+    sysm = make_system((n, n, n), nrays, seed=10_096, lam=lam, tol=1e-8)
+    out = {"grid": [n, n, n], "rays": nrays, "nnz": sysm.A.nnz}
+    for kind in ("jacobi", "learned"):
+        best = None
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if kind == "learned":
+                net.prepare(sysm.A, sysm.grid, lam)
+            _, info, _ = sysm.A.cgls(sysm.b, lam, tol, 20000, kind, network=net if kind == "learned" else None)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if best is None or dt < best[0]:
+                best = (dt, info)
+        out[kind] = {"seconds": round(best[0], 5), "iterations": best[1].iterations,
+                     "converged": bool(best[1].converged)}
+    out["learned_over_jacobi_time"] = round(out["learned"]["seconds"] / out["jacobi"]["seconds"], 4)
     return out
 
 
@@ -161,6 +245,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cg", action="store_true", help="skip the tomography-solve timing")
+    ap.add_argument("--no-rtm", action="store_true", help="skip the RTM imaging timing")
+    ap.add_argument("--no-grad", action="store_true", help="skip the 256^3 x 16-shot adjoint-gradient timing")
     ap.add_argument("--cg-only", action="store_true",
                     help="only the tomography-solve timing (one JSON object, for profiling the CGLS kernels)")
     ap.add_argument("--cg-kinds", default="jacobi,none,learned",
@@ -261,14 +347,27 @@ def main() -> int:
                      "avg_launch_us": round(launch_s * 1e6, 2), "traffic_source": traffic_src},
     }
 
+    if not args.no_rtm:
+        log(f"RTM imaging: {n}^3, {nt} steps, recorded traces back-propagated")
+        result["rtm"] = rtm_workload(fd, nt, w, tr)
+    del tr
+    if not args.no_grad:
+        vel_keep = fd.velocity.double().cpu().numpy() if not args.no_cpu_baseline else None
+        del fd
+        torch.cuda.empty_cache()
+        log("adjoint gradient: 256^3, 16 shots, 1500 steps")
+        result["gradient"] = gradient_workload(dev)
     if not args.no_cg:
         log("tomography solve: 200k rays, 64^3, CGLS to 1e-6")
         result["cg"] = cg_workload(dev)
 
     if not args.no_cpu_baseline:
-        vel_h = fd.velocity.double().cpu().numpy()
-        del fd
-        torch.cuda.empty_cache()
+        if args.no_grad:
+            vel_h = fd.velocity.double().cpu().numpy()
+            del fd
+            torch.cuda.empty_cache()
+        else:
+            vel_h = vel_keep
         steps = args.cpu_steps
         log(f"cpu baseline: FP64 oracle on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
         gcs, sec = cpu_baseline(args.cpu_threads, vel_h, n, steps)

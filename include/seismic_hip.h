@@ -84,6 +84,19 @@ int stp_fd_set_receivers(stp_fd_plan* plan, const int64_t* idx_host, int32_t nrc
  * wave_equation.cpp:120-124). */
 int stp_fd_set_sources(stp_fd_plan* plan, const int64_t* idx_host, int32_t nshots);
 
+/* Sponge form.  STP_SPONGE_REFERENCE (default) is apply_sponge_layer
+ * (wave_equation.cpp:262-297): the current and previous fields are damped
+ * before the buffer swap, which leaves u^{t+1} = 2u^t - D u^{t-1} + C L u^t;
+ * for smooth modes inside the sponge its recurrence has the root
+ * 1 + sqrt(1 - D) > 1, so long runs grow there until the amplitude clamp.
+ * STP_SPONGE_CERJAN (new; not in the reference) damps the next and current
+ * fields instead, u^{t+1} = D (2u^t - D u^{t-1} + C L u^t), which decays
+ * (root D); same memory traffic, one more multiply per cell.  The reverse-time
+ * pass (stp_fd_adjoint) follows the form set here. */
+#define STP_SPONGE_REFERENCE 0
+#define STP_SPONGE_CERJAN 1
+int stp_fd_set_sponge_mode(stp_fd_plan* plan, int32_t mode);
+
 /* C = v^2 dt^2 per with-halo cell, with the reference's substitutions
  * (non-finite or <= 0 velocity -> 1500, wave_equation.cpp:375-378) and its CFL
  * sub-cycling damp factor folded in (wave_equation.cpp:316-331, 383-387).

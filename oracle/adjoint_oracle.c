@@ -10,6 +10,10 @@
  * for J = 1/2 sum_{t,r} (d_r^t - o_r^t)^2, with phi = C lambda:
  *   phi^k = 2 phi^{k+1} - D phi^{k+2} + C L phi^{k+1} + C(x_r) e_r^{k-1}   (k = nt..1)
  *   dJ/dC = sum_k (phi^k / C) L u~^{k-1}
+ * In Cerjan sponge mode the forward step is the reference step times D, so
+ * the adjoint step is too, and the receiver term gains the factor D(x_r):
+ *   phi^k = D (2 phi^{k+1} - D phi^{k+2} + C L phi^{k+1}) + C(x_r) D(x_r) e_r^{k-1}
+ * with the same accumulation.
  * It keeps every forward state in memory (small grids only).  Its gradient is
  * pinned in the tests against central finite differences of J computed with
  * the forward oracle.
@@ -94,12 +98,14 @@ int orc_fd_adjoint(const orc_fd_cfg* c, const double* vel, const double* wavelet
             for (int64_t iy = c->hw; iy < nyh - c->hw; ++iy)
                 for (int64_t iz = c->hw; iz < nzh - c->hw; ++iz) {
                     const int64_t i = (ix * nyh + iy) * nzh + iz;
-                    pk[i] = 2.0 * p1[i] - D[i] * p2[i] + C[i] * lp[i];
+                    const double f = c->sponge_mode == 1 ? D[i] : 1.0;
+                    pk[i] = f * (2.0 * p1[i] - D[i] * p2[i] + C[i] * lp[i]);
                 }
         if (inj)
             for (int32_t r = 0; r < nrcv; ++r) {
                 const double e = inj[(int64_t)r * nt + t];
-                pk[rcv[r]] += inj_scale_c ? C[rcv[r]] * e : e;
+                const double f = c->sponge_mode == 1 ? D[rcv[r]] : 1.0;
+                pk[rcv[r]] += f * (inj_scale_c ? C[rcv[r]] * e : e);
             }
         const double* s = states + (size_t)t * n;
         if (snap_lap) { orc_fd_laplacian(c, s, snap); s = snap; }

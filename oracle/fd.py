@@ -16,7 +16,7 @@ class OrcCfg(C.Structure):
         ("hw", C.c_int32), ("order", C.c_int32),
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
         ("sponge_width", C.c_int32), ("sponge_damping", C.c_double),
-        ("amplitude_limit", C.c_double), ("cfl_safety_factor", C.c_double),
+        ("amplitude_limit", C.c_double), ("cfl_safety_factor", C.c_double), ("sponge_mode", C.c_int32),
     ]
 
 
@@ -40,11 +40,14 @@ def lib() -> C.CDLL:
     return _lib
 
 
+SPONGE_MODES = {"reference": 0, "cerjan": 1}
+
+
 def cfg(n, hw=4, order=8, h=(10.0, 10.0, 10.0), dt=1e-3, sponge_width=20, sponge_damping=0.015,
-        amplitude_limit=1e10, cfl_safety_factor=0.49) -> OrcCfg:
+        amplitude_limit=1e10, cfl_safety_factor=0.49, sponge="reference") -> OrcCfg:
     nx, ny, nz = n
     return OrcCfg(nx, ny, nz, hw, order, h[0], h[1], h[2], dt, sponge_width, sponge_damping, amplitude_limit,
-                  cfl_safety_factor)
+                  cfl_safety_factor, SPONGE_MODES[sponge])
 
 
 def _d(a):

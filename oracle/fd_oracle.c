@@ -8,6 +8,7 @@
  * It restates, loop for loop, the reference solver's forward stepping:
  *   - stencil pass with finite guards  : src/cpp/solver/wave_equation.cpp:299-405
  *   - sponge pass (current and previous): src/cpp/solver/wave_equation.cpp:262-297
+ *     (sponge_mode 1: the Cerjan variant, next and current; not in the reference)
  *   - buffer rotation                  : src/cpp/solver/wave_equation.cpp:56-76
  *   - health check + repair            : src/cpp/solver/wave_equation.cpp:407-433,
  *                                         src/cpp/core/fault_tolerance.cpp:72-157,198-257
@@ -247,7 +248,14 @@ int orc_fd_step(const orc_fd_cfg* c, real** bufs, const real* vel, orc_fault_sta
     const int64_t n = (c->nx + 2 * c->hw) * (c->ny + 2 * c->hw) * (c->nz + 2 * c->hw);
     int fd = orc_fd_stencil(c, bufs[0], bufs[1], bufs[2], vel);
     if (fd < 0) return fd;
-    orc_fd_sponge(c, bufs[0], bufs[1]);
+    /* reference: the current and previous fields are damped before the swap,
+     * so the damped current becomes the new previous and the damped previous
+     * is discarded; the recurrence u^{t+1} = 2u^t - D u^{t-1} + C L u^t then
+     * has the root 1 + sqrt(1 - D) > 1 for smooth modes inside the sponge
+     * (it grows).  Cerjan mode damps the next and current fields instead:
+     * u^{t+1} = D (2u^t - D u^{t-1} + C L u^t), whose smooth-mode root is D. */
+    if (c->sponge_mode == 1) orc_fd_sponge(c, bufs[2], bufs[0]);
+    else orc_fd_sponge(c, bufs[0], bufs[1]);
     real* t = bufs[1]; bufs[1] = bufs[0]; bufs[0] = t;   /* swap(previous, current) */
     t = bufs[0]; bufs[0] = bufs[2]; bufs[2] = t;         /* swap(current, next) */
     if (orc_count_nan_inf(bufs[0], n) > 0) {

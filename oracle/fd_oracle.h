@@ -12,6 +12,8 @@ typedef struct {
     double sponge_damping;
     double amplitude_limit;  /* reference default 1e10 */
     double cfl_safety_factor;/* reference default 0.49 */
+    int32_t sponge_mode;     /* 0: reference (damps current and previous before the swap);
+                                1: Cerjan (damps next and current: the stored fields both decay) */
 } orc_fd_cfg;
 
 typedef struct {

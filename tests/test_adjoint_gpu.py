@@ -14,12 +14,12 @@ HW = 4
 NT = 110
 
 
-def _setup(cuda, nshots=1, srcs=((12, 10, 3),)):
+def _setup(cuda, nshots=1, srcs=((12, 10, 3),), sponge="reference"):
     v_true = synthetic.smooth_random_velocity(*N, seed=1, sigma=3.0, vmin=1800, vmax=2600)
     v0 = synthetic.smooth_random_velocity(*N, seed=2, sigma=4.0, vmin=1900, vmax=2500)
     w = synthetic.ricker(25.0, 1e-3, NT)
     rcv = synthetic.surface_receivers(*N, HW, iz=1)
-    fd = FDPropagator(*N, order=8, sponge_width=6, nshots=nshots, device=cuda)
+    fd = FDPropagator(*N, order=8, sponge_width=6, nshots=nshots, device=cuda, sponge=sponge)
     fd.set_sources([fd.index(*s) for s in srcs])
     fd.set_receivers(rcv)
     return fd, v_true, v0, w, rcv
@@ -35,12 +35,13 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-def test_gradient_matches_fp64_oracle(cuda):
-    fd, v_true, v0, w, rcv = _setup(cuda)
+@pytest.mark.parametrize("sponge", ["reference", "cerjan"])
+def test_gradient_matches_fp64_oracle(sponge, cuda):
+    fd, v_true, v0, w, rcv = _setup(cuda, sponge=sponge)
     obs = _obs(fd, v_true, w)
     fd.set_velocity(v0)
     J, g, tr = fd.misfit_gradient(NT, w, obs)
-    c = ofd.cfg(N, hw=HW, order=8, sponge_width=6)
+    c = ofd.cfg(N, hw=HW, order=8, sponge_width=6, sponge=sponge)
     src = fd.index(12, 10, 3)
     v0h = synthetic.pad_halo(v0, HW)
     obs_np = obs[0].cpu().numpy().astype(np.float64)
@@ -87,13 +88,14 @@ def test_gradient_finite_differences_8_directions(cuda):
     assert max(errs) < 2e-2, errs
 
 
-def test_rtm_image_matches_oracle_and_finalize(cuda):
+@pytest.mark.parametrize("sponge", ["reference", "cerjan"])
+def test_rtm_image_matches_oracle_and_finalize(sponge, cuda):
     import torch
-    fd, v_true, v0, w, rcv = _setup(cuda)
+    fd, v_true, v0, w, rcv = _setup(cuda, sponge=sponge)
     data = _obs(fd, v_true, w)
     fd.set_velocity(v0)
     out = fd.reverse_time(NT, w, inject=data, snapshot="field", scale_by_c=False, illumination=True)
-    c = ofd.cfg(N, hw=HW, order=8, sponge_width=6)
+    c = ofd.cfg(N, hw=HW, order=8, sponge_width=6, sponge=sponge)
     img_o, ill_o, _ = oad.adjoint(c, synthetic.pad_halo(v0, HW), w, NT, fd.index(12, 10, 3), rcv,
                                   inj=data[0].cpu().numpy().astype(np.float64), scale_c=False, snap_lap=False,
                                   ill=True)

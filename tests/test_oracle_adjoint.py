@@ -2,6 +2,7 @@
 central finite differences of the forward oracle's misfit, and its image is
 linear in the injected traces.  CPU only."""
 import numpy as np
+import pytest
 
 from oracle import adjoint as oad
 from oracle import fd as ofd
@@ -12,8 +13,8 @@ HW = 4
 NT = 70
 
 
-def _case():
-    c = ofd.cfg(N, hw=HW, order=8, sponge_width=4)
+def _case(sponge="reference"):
+    c = ofd.cfg(N, hw=HW, order=8, sponge_width=4, sponge=sponge)
     v_true = synthetic.pad_halo(synthetic.smooth_random_velocity(*N, seed=1, sigma=2.5, vmin=1800, vmax=2600), HW)
     v0 = synthetic.pad_halo(synthetic.smooth_random_velocity(*N, seed=2, sigma=3.0, vmin=1900, vmax=2500), HW)
     w = synthetic.ricker(25.0, 1e-3, NT)
@@ -22,8 +23,9 @@ def _case():
     return c, v_true, v0, w, src, rcv
 
 
-def test_gradient_equals_finite_differences():
-    c, v_true, v0, w, src, rcv = _case()
+@pytest.mark.parametrize("sponge", ["reference", "cerjan"])
+def test_gradient_equals_finite_differences(sponge):
+    c, v_true, v0, w, src, rcv = _case(sponge)
     _, _, obs = oad.adjoint(c, v_true, w, NT, src, rcv)
     _, _, tr = oad.adjoint(c, v0, w, NT, src, rcv)
     acc, _, _ = oad.adjoint(c, v0, w, NT, src, rcv, inj=tr - obs, scale_c=True, snap_lap=True)

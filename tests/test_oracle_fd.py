@@ -200,3 +200,59 @@ def test_repair_field_nan_hand_case():
     d[13] = np.nan
     out = ofd.repair_field_nan(d, 4, 3, 3, 1, 0.0)
     assert out[13] == 22.0   # only interior neighbour is (2,1,1)
+
+
+def _sponge_run(sponge, n=32, sw=12, nt=800):
+    from seismic_engine import synthetic
+    c = ofd.cfg((n, n, n), hw=4, order=8, sponge_width=sw, sponge=sponge)
+    v = np.full(ofd.nh(c), 3000.0)
+    w = ofd.ricker(25.0, 1e-3, 1 / 25, nt)
+    cur = np.zeros(ofd.nh(c))
+    prev = np.zeros_like(cur)
+    src = synthetic.cell_index((n, n, n), 4, n // 2, n // 2, n // 2)
+    peaks = []
+    for k in range(0, nt, 100):
+        ofd.run(c, v, cur, prev, 100, src_idx=[src], src_w=w[None, k:k + 100])
+        peaks.append(np.abs(cur).max())
+    return peaks
+
+
+def test_reference_sponge_grows_and_cerjan_sponge_decays():
+    """The reference sponge damps the current and previous fields before the
+    swap, leaving u^{t+1} = 2u^t - D u^{t-1} + C L u^t: for smooth modes
+    inside the sponge the recurrence has the root 1 + sqrt(1 - D) > 1 and the
+    field grows long after the wave has left.  The Cerjan form (next and
+    current damped) has the root D < 1 there and decays."""
+    ref = _sponge_run("reference")
+    cer = _sponge_run("cerjan")
+    assert ref[-1] > 1e3 * ref[0]
+    assert cer[-1] < 0.1 * cer[0]
+    assert max(cer) < 2 * cer[0]
+
+
+def test_cerjan_step_damps_next_and_current():
+    """One step from the same state: the new previous field is D u in both
+    forms, the new current field of the Cerjan form is D times the reference
+    one, and without a sponge the two forms are the same scheme."""
+    n = (18, 16, 20)
+    out = {}
+    for mode in ("reference", "cerjan"):
+        c = ofd.cfg(n, hw=4, order=8, sponge_width=6, sponge=mode)
+        cur = np.zeros(ofd.nh(c))
+        cur[4:-4, 4:-4, 4:-4] = np.random.default_rng(8).standard_normal(n)
+        prev = np.zeros_like(cur)
+        prev[4:-4, 4:-4, 4:-4] = np.random.default_rng(9).standard_normal(n)
+        u0 = cur.copy()
+        ofd.run(c, np.full(ofd.nh(c), 2500.0), cur, prev, 1)
+        out[mode] = (cur, prev, u0)
+    d = ofd.sponge_factors(ofd.cfg(n, hw=4, order=8, sponge_width=6))
+    np.testing.assert_allclose(out["cerjan"][1], d * out["cerjan"][2], rtol=1e-15, atol=0)
+    np.testing.assert_array_equal(out["cerjan"][1], out["reference"][1])
+    np.testing.assert_allclose(out["cerjan"][0], d * out["reference"][0], rtol=1e-15, atol=0)
+    assert not np.array_equal(out["cerjan"][0], out["reference"][0])
+    runs = []
+    for mode in ("reference", "cerjan"):
+        c = ofd.cfg(n, hw=4, order=8, sponge_width=0, sponge=mode)
+        w = ofd.ricker(25.0, 1e-3, 1 / 25, 60)
+        runs.append(ofd.forward(c, np.full(ofd.nh(c), 2500.0), w, 60)[0])
+    np.testing.assert_array_equal(runs[0], runs[1])
