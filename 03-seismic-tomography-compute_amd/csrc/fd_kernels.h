@@ -54,7 +54,9 @@ struct FdStepArgs {
     int inj_scale_c;
 };
 
-enum { FD_MODE_PLAIN = 0, FD_MODE_SNAPSHOT = 1, FD_MODE_ACCUMULATE = 2 };
+// FD_MODE_SNAPSHOT is what callers pass (snap_lap picks the kind); the
+// kernel is compiled separately for the field (1) and Laplacian (3) kinds.
+enum { FD_MODE_PLAIN = 0, FD_MODE_SNAPSHOT = 1, FD_MODE_ACCUMULATE = 2, FD_MODE_SNAPSHOT_LAP = 3 };
 
 // Arguments of one two-step launch (fd_step2.hip).
 struct FdStep2Args {

@@ -314,7 +314,7 @@ fd_step_kernel(const FdStepArgs a) {
                 const float c = V::get(cc[r], j);
                 float nv = u + (fmaf(-d, p, u) + c * lap);
                 if (damp_next) nv *= ds;
-                if constexpr (MODE == FD_MODE_SNAPSHOT) V::set(lapv[r], j, lap);
+                if constexpr (MODE == FD_MODE_SNAPSHOT_LAP) V::set(lapv[r], j, lap);
                 bad |= !finitef(nv) && rvalid[r];
                 V::set(res[r], j, fminf(fmaxf(nv, -amp), amp));
             }
@@ -368,9 +368,12 @@ fd_step_kernel(const FdStepArgs a) {
                     __builtin_nontemporal_store(V::to_native(res[r]),
                                                 reinterpret_cast<typename Vec<VW>::N*>(P + pb + roff[r]));
                 else *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
-                if constexpr (MODE == FD_MODE_SNAPSHOT) {
+                if constexpr (MODE == FD_MODE_SNAPSHOT || MODE == FD_MODE_SNAPSHOT_LAP) {
                     float* snap = a.snap_out + (int64_t)shot * a.ncell;
-                    *reinterpret_cast<vec*>(snap + pb + roff[r]) = a.snap_lap ? lapv[r] : q[R][r];
+                    if constexpr (MODE == FD_MODE_SNAPSHOT_LAP)
+                        *reinterpret_cast<vec*>(snap + pb + roff[r]) = lapv[r];
+                    else
+                        *reinterpret_cast<vec*>(snap + pb + roff[r]) = q[R][r];
                 }
                 if constexpr (MODE == FD_MODE_ACCUMULATE) {
                     const float* S = a.acc_src + (int64_t)shot * a.ncell;
@@ -495,7 +498,12 @@ __global__ void fd_sponge_kernel(float* d, int nxh, int nyh, int nzh, int hw, in
 template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL = false>
 static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStream_t st) {
     constexpr int NT = (TZ / VW) * (TY / TYR);
-    if (mode == FD_MODE_SNAPSHOT)
+    // the snapshot kind is a compile-time mode: a run-time choice between the
+    // field and its Laplacian made the compiler spill the plane queue
+    if (mode == FD_MODE_SNAPSHOT && a0.snap_lap)
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_SNAPSHOT_LAP>), dim3(nblocks), dim3(NT),
+                           0, st, a0);
+    else if (mode == FD_MODE_SNAPSHOT)
         hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_SNAPSHOT>), dim3(nblocks), dim3(NT), 0,
                            st, a0);
     else if (mode == FD_MODE_ACCUMULATE)

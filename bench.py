@@ -42,39 +42,42 @@ def pmc_traffic(n: int) -> tuple[float | None, str | None]:
     return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
-def cpu_baseline(threads: int, vel_h, n: int, steps: int):
+def cpu_baseline(threads: int, vel_h, n: int, steps: int, sponge: str):
     """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
     on the benchmark workload itself: the same n^3 model, order 8, sponge,
-    centre Ricker source and receivers, for the first `steps` time steps."""
+    centre Ricker source and receivers, for the first `steps` time steps.
+    Returns the rate, the time and the final field and traces (the GPU's
+    first `steps` steps are compared with them)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    import numpy as np
     from oracle import fd as ofd
     from seismic_engine import synthetic
-    c = ofd.cfg((n, n, n), hw=4, order=8, sponge_width=20, sponge_damping=0.015)
+    c = ofd.cfg((n, n, n), hw=4, order=8, sponge_width=20, sponge_damping=0.015, sponge=sponge)
     w = synthetic.ricker(15.0, 1e-3, steps)
     rcv = synthetic.line_receivers(n, n, n, 4, iy=n // 2, step=max(1, n // 256))
     t0 = time.perf_counter()
-    _, _, _, res = ofd.forward(c, vel_h, w, steps, rcv=rcv)
+    cur, _, tr, res = ofd.forward(c, vel_h, w, steps, rcv=rcv)
     dt = time.perf_counter() - t0
     assert res.completed_successfully == 1
-    return float(n) ** 3 * steps / dt / 1e9, dt
+    return float(n) ** 3 * steps / dt / 1e9, dt, cur, tr
 
 
-def accuracy_check(dev, threads: int, steps: int = 500):
-    """FP32 GPU propagation against the FP64 oracle on the 128^3 two-layer
-    case (full 500 steps): relative L2 error of the final field and traces."""
+def accuracy_check(dev, threads: int, vel_int, sponge: str, case: str, steps: int = 500):
+    """FP32 GPU propagation against the FP64 oracle on a 128^3 model (full
+    `steps` steps, centre Ricker source, receivers at every surface cell):
+    relative L2 error of the final field and of the traces, and the largest
+    trace amplitude (a growing sponge would show there)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import numpy as np
     import torch
     from oracle import fd as ofd
     from seismic_engine import FDPropagator, synthetic
     n = (128, 128, 128)
-    c = ofd.cfg(n, hw=4, order=8, sponge_width=20, sponge_damping=0.015)
-    v = synthetic.pad_halo(synthetic.two_layer_velocity(*n, split_iz=64), 4)
+    c = ofd.cfg(n, hw=4, order=8, sponge_width=20, sponge_damping=0.015, sponge=sponge)
+    v = synthetic.pad_halo(vel_int, 4)
     w = synthetic.ricker(15.0, 1e-3, steps)
     rcv = synthetic.surface_receivers(*n, 4)
     cur_o, _, tr_o, res = ofd.forward(c, v, w, steps, rcv=rcv)
-    g = FDPropagator(*n, order=8, sponge_width=20, sponge_damping=0.015, device=dev)
+    g = FDPropagator(*n, order=8, sponge_width=20, sponge_damping=0.015, device=dev, sponge=sponge)
     g.set_velocity(v)
     g.set_sources([g.index(64, 64, 64)])
     g.set_receivers(rcv)
@@ -82,7 +85,8 @@ def accuracy_check(dev, threads: int, steps: int = 500):
     torch.cuda.synchronize()
     e = float(np.linalg.norm(g.cur[0].cpu().numpy() - cur_o) / np.linalg.norm(cur_o))
     et = float(np.linalg.norm(trg[0].cpu().numpy() - tr_o) / np.linalg.norm(tr_o))
-    return {"case": f"128^3 two-layer, {steps} steps", "field_rel_l2_vs_fp64": e, "traces_rel_l2_vs_fp64": et}
+    return {"case": f"{case}, 128^3, {steps} steps, {sponge} sponge", "field_rel_l2_vs_fp64": e,
+            "traces_rel_l2_vs_fp64": et, "trace_max_abs_fp64": float(np.abs(tr_o).max())}
 
 
 def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: float = 1e-6, reps: int = 3,
@@ -175,7 +179,7 @@ def rtm_workload(fd, nt: int, w, traces):
                       "image/illumination accumulation"}
 
 
-def gradient_workload(dev, n: int = 256, nshots: int = 16, nt: int = 1500):
+def gradient_workload(dev, n: int = 256, nshots: int = 16, nt: int = 1500, sponge: str = "cerjan"):
     """Adjoint-state gradient: n^3 grid, 16 shots on a 4 x 4 surface grid
     (iz = 2), receivers every 4th surface cell, smooth random true and
     starting velocities (seeds 1 and 2, sigma 20 cells, 1500-4500 m/s), L2
@@ -188,11 +192,13 @@ def gradient_workload(dev, n: int = 256, nshots: int = 16, nt: int = 1500):
     w = torch.as_tensor(synthetic.ricker(15.0, 1e-3, nt), dtype=torch.float32, device=dev)
     pos = [n // 8 * (1 + 2 * k) for k in range(4)]
     rcv = synthetic.surface_receivers(n, n, n, hw)[::1].reshape(n, n)[::4, ::4].reshape(-1)
-    fd = FDPropagator(n, n, n, order=8, sponge_width=20, sponge_damping=0.015, nshots=nshots, device=dev)
+    fd = FDPropagator(n, n, n, order=8, sponge_width=20, sponge_damping=0.015, nshots=nshots, device=dev,
+                      sponge=sponge)
     fd.set_sources([fd.index(ix, iy, 2) for ix in pos for iy in pos])
     fd.set_receivers(rcv)
     fd.set_velocity(synthetic.smooth_random_velocity(n, n, n, seed=1, sigma=20.0, device=dev, as_tensor=True))
     obs = fd.propagate(nt, w).clone()
+    obs_max = float(obs.abs().max())
     fd.reset()
     fd.set_velocity(synthetic.smooth_random_velocity(n, n, n, seed=2, sigma=20.0, device=dev, as_tensor=True))
     torch.cuda.synchronize()
@@ -204,8 +210,9 @@ def gradient_workload(dev, n: int = 256, nshots: int = 16, nt: int = 1500):
     cells = float(n) ** 3 * nshots * nt
     del fd, obs, g
     torch.cuda.empty_cache()
-    return {"grid": [n, n, n], "shots": nshots, "time_steps": nt, "receivers": int(rcv.size), "seconds": round(sec, 3),
-            "shot_gcell_steps_per_s": round(cells / sec / 1e9, 2), "misfit": J, "finite": ok}
+    return {"grid": [n, n, n], "shots": nshots, "time_steps": nt, "receivers": int(rcv.size), "sponge": sponge,
+            "seconds": round(sec, 3), "shot_gcell_steps_per_s": round(cells / sec / 1e9, 2), "misfit": J,
+            "obs_max_abs": obs_max, "finite": ok}
 
 
 def learned_heldout(dev, n: int = 96, lam: float = 1e-2, tol: float = 1e-6, net=None):
@@ -246,6 +253,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cg", action="store_true", help="skip the tomography-solve timing")
     ap.add_argument("--no-rtm", action="store_true", help="skip the RTM imaging timing")
+    ap.add_argument("--sponge", default="cerjan", choices=("cerjan", "reference"),
+                    help="sponge form of the FD, RTM and gradient workloads (the reference form grows inside the "
+                         "sponge on these 1500-4500 m/s models; the kernel's memory traffic is the same)")
     ap.add_argument("--no-grad", action="store_true", help="skip the 256^3 x 16-shot adjoint-gradient timing")
     ap.add_argument("--cg-only", action="store_true",
                     help="only the tomography-solve timing (one JSON object, for profiling the CGLS kernels)")
@@ -280,7 +290,7 @@ def main() -> int:
     n, nt, hw = args.n, args.nt, 4
     log(f"building {n}^3 velocity (seed 2026, sigma 20) on the GPU")
     vel = synthetic.smooth_random_velocity(n, n, n, seed=2026, sigma=20.0, device=dev, as_tensor=True)
-    fd = FDPropagator(n, n, n, order=8, sponge_width=20, sponge_damping=0.015, device=dev)
+    fd = FDPropagator(n, n, n, order=8, sponge_width=20, sponge_damping=0.015, device=dev, sponge=args.sponge)
     fd.set_velocity(vel)
     del vel
     if not fd.verify_cfl():
@@ -317,6 +327,13 @@ def main() -> int:
     prop_ms = [a.elapsed_time(b) for a, b in ev_prop]
     log(f"timed {args.steps} propagations: wall {wall:.3f} s, propagate {prop_ms} ms")
     assert torch.isfinite(tr).all()
+    tr_max = float(tr.abs().max())
+    field_max = float(fd.cur.abs().max())
+    spot = None
+    if not args.no_cpu_baseline:
+        # the first cpu-steps steps again, for the 512^3 spot check against the FP64 oracle
+        fd.reset()
+        spot = (fd.propagate(args.cpu_steps, w[: args.cpu_steps]).cpu().numpy(), fd.cur[0].cpu().numpy())
 
     cells = float(n) ** 3
     value = cells * nt * args.steps / wall / 1e9
@@ -340,7 +357,9 @@ def main() -> int:
         "config": {"workload": f"fd8_forward_{n}cube_{nt}steps_{len(rcv)}rcv", "grid": [n, n, n],
                    "order": 8, "time_steps": nt, "receivers": int(len(rcv)), "sponge": [20, 0.015],
                    "velocity": "smooth random seed 2026 sigma 20, 1500-4500 m/s", "source": "Ricker 15 Hz centre",
-                   "step": "one full forward propagation"},
+                   "sponge_form": args.sponge, "step": "one full forward propagation"},
+        "wavefield": {"trace_max_abs": tr_max, "final_field_max_abs": field_max,
+                      "amplitude_limit_reached": bool(field_max >= 0.99e10)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "fd_step_kernel<4,4,128,16,1,256>", "bytes_per_cell_update": BYTES_PER_CELL,
@@ -350,13 +369,14 @@ def main() -> int:
     if not args.no_rtm:
         log(f"RTM imaging: {n}^3, {nt} steps, recorded traces back-propagated")
         result["rtm"] = rtm_workload(fd, nt, w, tr)
+        result["rtm"]["sponge"] = args.sponge
     del tr
     if not args.no_grad:
         vel_keep = fd.velocity.double().cpu().numpy() if not args.no_cpu_baseline else None
         del fd
         torch.cuda.empty_cache()
         log("adjoint gradient: 256^3, 16 shots, 1500 steps")
-        result["gradient"] = gradient_workload(dev)
+        result["gradient"] = gradient_workload(dev, sponge=args.sponge)
     if not args.no_cg:
         log("tomography solve: 200k rays, 64^3, CGLS to 1e-6")
         result["cg"] = cg_workload(dev)
@@ -370,15 +390,24 @@ def main() -> int:
             vel_h = vel_keep
         steps = args.cpu_steps
         log(f"cpu baseline: FP64 oracle on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
-        gcs, sec = cpu_baseline(args.cpu_threads, vel_h, n, steps)
+        gcs, sec, cur_o, tr_o = cpu_baseline(args.cpu_threads, vel_h, n, steps, args.sponge)
         del vel_h
+        tr_g, cur_g = spot
+        result["spot_check_512"] = {
+            "steps": steps, "field_rel_l2_vs_fp64": float(np.linalg.norm(cur_g - cur_o) / np.linalg.norm(cur_o)),
+            "traces_rel_l2_vs_fp64": float(np.linalg.norm(tr_g[0] - tr_o) / max(np.linalg.norm(tr_o), 1e-300))}
+        del cur_o, tr_o, cur_g, tr_g
         result["cpu_baseline"] = {
             "value": round(gcs, 5), "unit": "Gcell-updates/s", "cores": args.cpu_threads, "kind": "port",
             "sample": f"FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x) on the same "
                       f"{n}^3 smooth random model, order 8, sponge 20, {len(rcv)} receivers: first {steps} of "
                       f"{nt} steps, {sec:.1f} s"}
-        log("accuracy: 128^3 two-layer, 500 steps, FP32 GPU vs FP64 oracle")
-        result["accuracy"] = accuracy_check(dev, args.cpu_threads)
+        log("accuracy: 128^3, 500 steps, FP32 GPU vs FP64 oracle")
+        result["accuracy"] = [
+            accuracy_check(dev, args.cpu_threads, synthetic.two_layer_velocity(128, 128, 128, split_iz=64),
+                           "reference", "two-layer 1500/2500 m/s (split iz = 64)"),
+            accuracy_check(dev, args.cpu_threads, synthetic.smooth_random_velocity(128, 128, 128, seed=2026, sigma=5.0),
+                           args.sponge, "smooth random 1500-4500 m/s, seed 2026, sigma 5 cells")]
     print(json.dumps(result), flush=True)
     return 0
 
