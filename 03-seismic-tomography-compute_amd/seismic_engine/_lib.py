@@ -100,6 +100,8 @@ SIGNATURES: dict[str, list] = {
     "stp_lpc_adam_step": [_vp, _f64, _vp],
     "stp_conv3d_bf16": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _i32, _vp, _vp],
     "stp_conv3d_bf16_bwd": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
+    "stp_tt_forward": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "stp_tt_gradient": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

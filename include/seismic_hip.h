@@ -319,6 +319,24 @@ int stp_conv3d_bf16(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t cou
 int stp_conv3d_bf16_bwd(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t cout, const uint16_t* in_dev,
                         const float* w_dev, const uint16_t* dout_dev, float* din_dev, float* dw_dev, void* stream);
 
+/* ---------------------------------------------------------------- travel-time tomography
+ * The straight-ray operator of the reference's Python inversion
+ * (src/python/seismic_tomography/inversion.py:329-383 _forward_model and
+ * :385-433 _compute_gradient), FP64 on the device.  Pick i joins src_xyz[3i..]
+ * and sta_xyz[3i..] (physical coordinates, device arrays); the model is
+ * n[0] x n[1] x n[2] cells (x-major, z fastest) of size h[] from origin[]
+ * (host arrays).  vp_dev holds the velocity per cell. */
+/* times_dev[i] = sum over the pick's samples of ds / v (reference order and
+ * FP64 arithmetic, so the same value bit for bit) */
+int stp_tt_forward(int64_t npicks, const double* src_xyz_dev, const double* sta_xyz_dev, const int64_t* n,
+                   const double* origin, const double* h, const double* vp_dev, double* times_dev, void* stream);
+/* grad_dev[cell] = sum_i r_i (-ds / v^2) w_i over the samples in the cell
+ * (overwritten; r = weighted residuals as the reference passes them, weights
+ * optional = 1) */
+int stp_tt_gradient(int64_t npicks, const double* src_xyz_dev, const double* sta_xyz_dev, const int64_t* n,
+                    const double* origin, const double* h, const double* vp_dev, const double* weighted_residuals_dev,
+                    const double* weights_dev, double* grad_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
