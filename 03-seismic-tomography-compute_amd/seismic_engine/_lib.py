@@ -102,6 +102,10 @@ SIGNATURES: dict[str, list] = {
     "stp_conv3d_bf16_bwd": [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp],
     "stp_tt_forward": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "stp_tt_gradient": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "stp_gradient_tikhonov_f64": [_i64, _i64, _i64, _f64, _vp, _vp, _vp],
+    "stp_gradient_normalize_f64": [_i64, _vp, C.POINTER(_f64), _vp],
+    "stp_depth_preconditioner_f64": [_i64, _i64, _i64, _i64, _f64, _vp, _vp],
+    "stp_gradient_precondition_f64": [_i64, _vp, _vp, _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

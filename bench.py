@@ -242,6 +242,60 @@ def learned_heldout(dev, n: int = 96, lam: float = 1e-2, tol: float = 1e-6, net=
     return out
 
 
+def traveltime_workload(dev, n: int = 64, npicks: int = 200_000, h: float = 10.0, cpu_sample: int = 2000):
+    """Travel-time tomography (the reference's straight-ray sampled operator):
+    npicks source/station pairs on opposite faces of an n^3 model of h-metre
+    cells (the 200k-ray tomography geometry), smooth random velocity; one
+    forward model, one gradient, and one iteration of the inversion loop
+    (forward, gradient, smoothing, Tikhonov term, normalisation, line search).
+    The reference's own pure-Python loops (restated in oracle/traveltime.py)
+    are timed on a sample of the picks as the CPU baseline."""
+    import numpy as np
+    import torch
+    from oracle import traveltime as ott
+    from seismic_engine import synthetic
+    from seismic_engine.inversion import TravelTimeInversion
+    from seismic_engine.traveltime import Geometry, TravelTimeOperator
+    pts = synthetic.ray_endpoints(npicks, (n, n, n), h=h, seed=2026)
+    src, sta = pts[:, 0].copy(), pts[:, 1].copy()
+    geo = Geometry(n, n, n, dx=h, dy=h, dz=h)
+    vp = synthetic.smooth_random_velocity(n, n, n, seed=2027, sigma=5.0).reshape(-1)
+    true = synthetic.smooth_random_velocity(n, n, n, seed=2028, sigma=5.0).reshape(-1)
+    op = TravelTimeOperator(geo, src, sta, device=dev)
+    obs = op.forward(true)
+    t = op.forward(vp)
+    r = op.weighted_residuals(t, obs)
+    g = op.gradient(vp, r)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t = op.forward(vp)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = op.gradient(vp, r)
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / reps
+    inv = TravelTimeInversion(op, obs, vp, max_iterations=1, step_length=50.0, smoothing_sigma=2.0, misfit_type="l2",
+                              descent=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    inv.run()
+    torch.cuda.synchronize()
+    t_it = time.perf_counter() - t0
+    samples = int(sum(max(int(np.linalg.norm(a - b) / (h * 5)), 2) for a, b in zip(src[:2000], sta[:2000])))
+    t0 = time.perf_counter()
+    ott.forward(src[:cpu_sample], sta[:cpu_sample], (n, n, n), (0.0, 0.0, 0.0), (h, h, h), vp)
+    t_cpu = time.perf_counter() - t0
+    return {"picks": npicks, "grid": [n, n, n], "samples_per_pick": round(samples / 2000, 2),
+            "forward_ms": round(t_fwd * 1e3, 3), "gradient_ms": round(t_grad * 1e3, 3),
+            "inversion_iteration_ms": round(t_it * 1e3, 2),
+            "cpu_reference_loop_forward_ms_per_200k_picks": round(t_cpu * npicks / cpu_sample * 1e3, 1),
+            "cpu_sample": f"pure-Python restatement of _forward_model on {cpu_sample} picks, 1 core, scaled"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +311,7 @@ def main() -> int:
                     help="sponge form of the FD, RTM and gradient workloads (the reference form grows inside the "
                          "sponge on these 1500-4500 m/s models; the kernel's memory traffic is the same)")
     ap.add_argument("--no-grad", action="store_true", help="skip the 256^3 x 16-shot adjoint-gradient timing")
+    ap.add_argument("--no-tt", action="store_true", help="skip the travel-time tomography timing")
     ap.add_argument("--cg-only", action="store_true",
                     help="only the tomography-solve timing (one JSON object, for profiling the CGLS kernels)")
     ap.add_argument("--cg-kinds", default="jacobi,none,learned",
@@ -377,6 +432,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("adjoint gradient: 256^3, 16 shots, 1500 steps")
         result["gradient"] = gradient_workload(dev, sponge=args.sponge)
+    if not args.no_tt:
+        log("travel-time tomography: 200k picks, 64^3")
+        result["traveltime"] = traveltime_workload(dev)
     if not args.no_cg:
         log("tomography solve: 200k rays, 64^3, CGLS to 1e-6")
         result["cg"] = cg_workload(dev)

@@ -337,6 +337,23 @@ int stp_tt_gradient(int64_t npicks, const double* src_xyz_dev, const double* sta
                     const double* origin, const double* h, const double* vp_dev, const double* weighted_residuals_dev,
                     const double* weights_dev, double* grad_dev, void* stream);
 
+/* ---------------------------------------------------------------- gradient post-processing
+ * GradientComputer of the reference's Python inversion
+ * (src/python/seismic_tomography/gradient.py:77-139), FP64, x-major arrays. */
+/* apply_regularization (:112-126): g -= lambda * (sum over axes of the second
+ * difference of the velocity, 0 at an axis' first and last index) */
+int stp_gradient_tikhonov_f64(int64_t nx, int64_t ny, int64_t nz, double lambda, const double* velocity_dev,
+                              double* gradient_dev, void* stream);
+/* normalize_gradient (:128-139): g /= max|g| when max|g| > 1e-30; the max
+ * goes to *max_abs_host when given (then the call synchronises `stream`) */
+int stp_gradient_normalize_f64(int64_t n, double* gradient_dev, double* max_abs_host, void* stream);
+/* build_depth_preconditioner (:96-110): 0 above water_depth_iz, else
+ * 1 / (1 + (iz - water_depth_iz) * dz / 10000) */
+int stp_depth_preconditioner_f64(int64_t nx, int64_t ny, int64_t nz, int64_t water_depth_iz, double dz,
+                                 double* out_dev, void* stream);
+/* apply_preconditioning (:79-94): g *= weights */
+int stp_gradient_precondition_f64(int64_t n, const double* weights_dev, double* gradient_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
