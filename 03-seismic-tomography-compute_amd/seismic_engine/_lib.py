@@ -106,6 +106,12 @@ SIGNATURES: dict[str, list] = {
     "stp_gradient_normalize_f64": [_i64, _vp, C.POINTER(_f64), _vp],
     "stp_depth_preconditioner_f64": [_i64, _i64, _i64, _i64, _f64, _vp, _vp],
     "stp_gradient_precondition_f64": [_i64, _vp, _vp, _vp],
+    "stp_terrain_bathymetry_synthetic_f64": [_i64, _i64, _f64, _f64, _f64, _f64, _f64, _vp, _vp],
+    "stp_terrain_velocity_mask_f64": [_vp, _vp, _vp, _f64, _f64, _vp, _vp],
+    "stp_terrain_boundary_mask": [_vp, _vp, _vp, _vp, _vp],
+    "stp_terrain_free_surface_f64": [_i64, _i64, _i64, _i64, _vp, _vp],
+    "stp_terrain_seafloor_f64": [_vp, _vp, _vp, _vp, _i64, _vp],
+    "stp_terrain_absorbing_f64": [_i64, _i64, _i64, _i64, _i32, _f64, _vp, _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

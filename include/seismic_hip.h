@@ -354,6 +354,31 @@ int stp_depth_preconditioner_f64(int64_t nx, int64_t ny, int64_t nz, int64_t wat
 /* apply_preconditioning (:79-94): g *= weights */
 int stp_gradient_precondition_f64(int64_t n, const double* weights_dev, double* gradient_dev, void* stream);
 
+/* ---------------------------------------------------------------- terrain (bathymetry) boundary
+ * TerrainBoundary of the reference's solver (src/cpp/solver/boundary.cpp),
+ * FP64 with-halo x-major fields, bathymetry [nx_global][ny_global] (device
+ * arrays).  ext (host, 9 values): nx/ny/nz with halo, halo width, global
+ * offsets x/y/z of the local block, nx_global, ny_global; geo (host, 3
+ * values): water depth, dz, origin_z. */
+/* generate_synthetic_bathymetry (:33-54) */
+int stp_terrain_bathymetry_synthetic_f64(int64_t nx, int64_t ny, double dx, double dy, double water_depth,
+                                         double gradient, double roughness, double* bathy_dev, void* stream);
+/* update_velocity_mask (:150-178) */
+int stp_terrain_velocity_mask_f64(const int64_t* ext, const double* geo, const double* bathy_dev, double water_vp,
+                                  double sediment_vp, double* velocity_dev, void* stream);
+/* build_boundary_mask (:180-204): 1 subsurface, 2 water, 0 halo */
+int stp_terrain_boundary_mask(const int64_t* ext, const double* geo, const double* bathy_dev, uint8_t* mask_dev,
+                              void* stream);
+/* apply_free_surface_bc (:80-92) */
+int stp_terrain_free_surface_f64(int64_t nxh, int64_t nyh, int64_t nzh, int64_t hw, double* field_dev, void* stream);
+/* apply_seafloor_bc (:94-114); field_size bounds the flat index as the
+ * reference's field.size() does */
+int stp_terrain_seafloor_f64(const int64_t* ext, const double* geo, const double* bathy_dev, double* field_dev,
+                             int64_t field_size, void* stream);
+/* apply_absorbing_bc (:116-148) */
+int stp_terrain_absorbing_f64(int64_t nxh, int64_t nyh, int64_t nzh, int64_t hw, int32_t sponge_width, double damping,
+                              double* field_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,7 +48,7 @@ def build_ref() -> list[Path]:
     out = REF_OUT / "libref.so"
     tmp = out.with_suffix(".so.tmp")
     srcs = [cpp / "sparse" / "csr_matrix.cpp", cpp / "rtm" / "rtm_engine.cpp", cpp / "core" / "config.cpp",
-            cpp / "core" / "grid.cpp", HERE / "ref_shim.cpp"]
+            cpp / "core" / "grid.cpp", cpp / "solver" / "boundary.cpp", HERE / "ref_shim.cpp"]
     if not all(s.exists() for s in srcs):
         return []
     # rtm_engine.h uses std::function without including <functional>

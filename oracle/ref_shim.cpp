@@ -11,7 +11,9 @@
 //   * RTMEngine::apply_image_condition / finalize_image / reset_image
 //     (src/cpp/rtm/rtm_engine.cpp:24-57, 170-190);
 //   * load_config (src/cpp/core/config.cpp:9-96);
-//   * CartesianGrid index math (src/cpp/core/grid.cpp:13-49).
+//   * CartesianGrid index math (src/cpp/core/grid.cpp:13-49);
+//   * TerrainBoundary (src/cpp/solver/boundary.cpp:8-204) on a single-block
+//     grid.
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -20,6 +22,7 @@
 
 #include "core/config.h"
 #include "core/grid.h"
+#include "solver/boundary.h"
 #include "rtm/rtm_engine.h"
 #include "sparse/csr_matrix.h"
 
@@ -110,6 +113,40 @@ int64_t ref_grid_index_with_halo(int64_t nx, int64_t ny, int64_t nz, int hw, int
     const int coords[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
     CartesianGrid g(p, coords, dims);
     return g.local_index_with_halo(ix, iy, iz);
+}
+
+// TerrainBoundary on an nx x ny x nz grid (halo hw, spacing h, origin o):
+// bathymetry (synthetic when gen != 0), then the velocity mask, the boundary
+// mask and the three boundary conditions applied to copies of `field`.
+void ref_terrain(int64_t nx, int64_t ny, int64_t nz, int hw, const double* h, const double* o, double water_depth,
+                 int gen, double gradient, double roughness, int64_t npts, const double* pts, double water_vp,
+                 double sediment_vp, int sponge_width, double damping, const double* field, double* bathy_out,
+                 double* vel_out, uint8_t* mask_out, double* free_out, double* sea_out, double* abs_out) {
+    GridParams p{nx, ny, nz, h[0], h[1], h[2], o[0], o[1], o[2], hw};
+    const int coords[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
+    CartesianGrid g(p, coords, dims);
+    TerrainBoundary t(g, water_depth);
+    if (gen) t.generate_synthetic_bathymetry(gradient, roughness);
+    std::vector<BathymetryPoint> bp;
+    for (int64_t k = 0; k < npts; ++k) bp.push_back({pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]});
+    if (npts) t.set_bathymetry_data(bp);
+    std::memcpy(bathy_out, t.bathymetry().data(), sizeof(double) * nx * ny);
+    const LocalGrid& loc = g.local();
+    const int64_t n = loc.nx_with_halo * loc.ny_with_halo * loc.nz_with_halo;
+    std::vector<double> v(field, field + n);
+    t.update_velocity_mask(v, water_vp, sediment_vp);
+    std::memcpy(vel_out, v.data(), sizeof(double) * n);
+    t.build_boundary_mask(loc, hw);
+    std::memcpy(mask_out, t.mask().data(), n);
+    std::vector<double> f(field, field + n);
+    t.apply_free_surface_bc(f, loc, hw);
+    std::memcpy(free_out, f.data(), sizeof(double) * n);
+    f.assign(field, field + n);
+    t.apply_seafloor_bc(f, loc, hw);
+    std::memcpy(sea_out, f.data(), sizeof(double) * n);
+    f.assign(field, field + n);
+    t.apply_absorbing_bc(f, loc, hw, sponge_width, damping);
+    std::memcpy(abs_out, f.data(), sizeof(double) * n);
 }
 
 }  // extern "C"

@@ -24,6 +24,7 @@
 #include "core/grid.h"
 #include "core/mpi_context.h"
 #include "rtm/rtm_engine.h"
+#include "solver/boundary.h"
 #include "solver/wave_equation.h"
 #include "sparse/csr_matrix.h"
 #include "sparse/sparse_ops.h"
@@ -284,11 +285,45 @@ int cmd_health(const std::string& dir) {
     return 0;
 }
 
+// TerrainBoundary on the grid of params.txt (synthetic bathymetry plus the
+// points of pts.f64), then the velocity mask, the boundary mask and the three
+// boundary conditions on copies of field.f64
+int cmd_terrain(const std::string& dir) {
+    const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
+    const int hw = (int)param(dir, "hw");
+    GridParams gp{nx, ny, nz, param(dir, "dx"), param(dir, "dy"), param(dir, "dz"), 0.0, 0.0, 0.0, hw};
+    CartesianGrid grid(gp);
+    TerrainBoundary t(grid, param(dir, "water"));
+    t.generate_synthetic_bathymetry(param(dir, "gradient"), param(dir, "roughness"));
+    const std::vector<double> pts = read<double>(dir + "/pts.f64");
+    std::vector<BathymetryPoint> bp;
+    for (size_t k = 0; k + 2 < pts.size(); k += 3) bp.push_back({pts[k], pts[k + 1], pts[k + 2]});
+    t.set_bathymetry_data(bp);
+    write(dir + "/bathy.f64", t.bathymetry());
+    const std::vector<double> field = read<double>(dir + "/field.f64");
+    std::vector<double> v(field);
+    t.update_velocity_mask(v, 1500.0, 1800.0);
+    write(dir + "/vel.f64", v);
+    t.build_boundary_mask(grid.local(), hw);
+    write(dir + "/mask.u8", t.mask());
+    std::vector<double> f(field);
+    t.apply_free_surface_bc(f, grid.local(), hw);
+    write(dir + "/free.f64", f);
+    f = field;
+    t.apply_seafloor_bc(f, grid.local(), hw);
+    write(dir + "/sea.f64", f);
+    f = field;
+    t.apply_absorbing_bc(f, grid.local(), hw, 5, 0.3);
+    write(dir + "/abs.f64", f);
+    std::printf("seafloor_index %lld water %d\n", (long long)t.seafloor_index(3, 4), t.is_water(3, 4, 0) ? 1 : 0);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: dropin_driver config|grid|fd|sparse|rtm|health [arg]\n");
+        std::fprintf(stderr, "usage: dropin_driver config|grid|fd|sparse|rtm|health|terrain [arg]\n");
         return 2;
     }
     const std::string cmd = argv[1], arg = argc > 2 ? argv[2] : "";
@@ -299,6 +334,7 @@ int main(int argc, char** argv) {
         if (cmd == "sparse") return cmd_sparse(arg);
         if (cmd == "rtm") return cmd_rtm(arg);
         if (cmd == "health") return cmd_health(arg);
+        if (cmd == "terrain") return cmd_terrain(arg);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

@@ -170,3 +170,28 @@ def test_fault_tolerance_on_device_matches_oracle(tmp_path, cuda):
     assert "recovered 1" in text
     rec = f64(tmp_path, "recovered.f64")
     assert np.all(np.isfinite(rec))
+
+
+def test_terrain_boundary_matches_reference(tmp_path, cuda):
+    """The C++ TerrainBoundary over the C ABI against the reference's own
+    boundary.cpp (oracle/_ref): masks, velocities and boundary conditions bit
+    for bit, the synthetic bathymetry within 1e-15."""
+    from test_terrain import HW, H, N, REFLIB, _field, ref_terrain
+    if not REFLIB.exists():
+        pytest.skip("reference build (oracle/_ref) not present")
+    field = _field(2)
+    field.tofile(tmp_path / "field.f64")
+    pts = np.array([[260.0, 310.0, 123.0], [1000.0, 20.0, 4550.0]])
+    pts.tofile(tmp_path / "pts.f64")
+    text = run("terrain", tmp_path, nx=N[0], ny=N[1], nz=N[2], hw=HW, dx=H[0], dy=H[1], dz=H[2], water=2000.0,
+               gradient=0.08, roughness=300.0)
+    ref = ref_terrain(gen=True, pts=pts, water=2000.0, field=field)
+    b = f64(tmp_path, "bathy.f64")
+    assert np.abs(b - ref["bathy"]).max() <= 1e-15 * np.abs(ref["bathy"]).max()
+    if np.array_equal(b, ref["bathy"]):
+        assert np.array_equal(f64(tmp_path, "vel.f64"), ref["vel"])
+        assert np.array_equal(np.fromfile(tmp_path / "mask.u8", np.uint8), ref["mask"])
+        assert np.array_equal(f64(tmp_path, "sea.f64"), ref["sea"])
+    assert np.array_equal(f64(tmp_path, "free.f64"), ref["free"])
+    assert np.array_equal(f64(tmp_path, "abs.f64"), ref["absorb"])
+    assert f"seafloor_index {int(b[3 * N[1] + 4] / H[2])}" in text
