@@ -2,6 +2,8 @@
 // image and illumination on the GPU.
 #include "rtm/rtm_engine.h"
 
+#include <chrono>
+#include <cmath>
 #include <stdexcept>
 
 #include "device.h"
@@ -53,6 +55,44 @@ void RTMEngine::accumulate_device_f32(const float* image, const float* illum) {
     if (image) dev::check(stp_accumulate_f32_f64(total_cells(), image, image_dev_, nullptr), "RTMEngine accumulate");
     if (illum) dev::check(stp_accumulate_f32_f64(total_cells(), illum, illum_dev_, nullptr), "RTMEngine accumulate");
     host_valid_ = false;
+}
+
+void RTMEngine::extract_slices(const double* fwd, const double* adj, const double* image_condition, int64_t ny,
+                               int64_t nz, int64_t timestep) {
+    if (!slice_cb_ || slice_specs_.empty()) return;
+    const double* ic = image_condition ? image_condition : result().image.data();
+    const int64_t hw = halo_width_, nynz = ny * nz;
+    auto fin = [](double v) { return std::isfinite(v) ? v : 0.0; };
+    for (const SliceSpec& spec : slice_specs_) {
+        if (spec.axis == SliceAxis::VOLUME) continue;
+        const int a = static_cast<int>(spec.axis);
+        const int64_t idx = spec.index >= 0 ? spec.index : (a == 0 ? nx_ : (a == 1 ? ny_ : nz_)) / 2;
+        CapturedSlice sl;
+        sl.timestep = timestep;
+        sl.axis = spec.axis;
+        sl.index = idx;
+        sl.dim0 = a == 0 ? ny_ : nx_;
+        sl.dim1 = a == 2 ? ny_ : nz_;
+        const int64_t n = sl.dim0 * sl.dim1;
+        sl.forward_slice.resize(static_cast<size_t>(n));
+        sl.adjoint_slice.resize(static_cast<size_t>(n));
+        sl.ic_slice.resize(static_cast<size_t>(n));
+        for (int64_t p = 0; p < sl.dim0; ++p)
+            for (int64_t q = 0; q < sl.dim1; ++q) {
+                int64_t src, ici;
+                if (a == 0) { src = (idx + hw) * nynz + (p + hw) * nz + (q + hw); ici = p * nz_ + q; }
+                else if (a == 1) { src = (p + hw) * nynz + (idx + hw) * nz + (q + hw); ici = p * nz_ + q; }
+                else { src = (p + hw) * nynz + (q + hw) * nz + (idx + hw); ici = p * ny_ + q; }
+                const size_t d = static_cast<size_t>(p * sl.dim1 + q);
+                sl.forward_slice[d] = fin(fwd[src]);
+                sl.adjoint_slice[d] = fin(adj[src]);
+                sl.ic_slice[d] = fin(ic[ici]);
+            }
+        sl.capture_timestamp =
+            std::chrono::duration<double>(std::chrono::high_resolution_clock::now().time_since_epoch()).count();
+        slice_cb_(sl);
+        slices_captured_.fetch_add(1, std::memory_order_release);
+    }
 }
 
 void RTMEngine::finalize_image() {

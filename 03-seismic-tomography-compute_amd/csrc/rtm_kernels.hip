@@ -104,3 +104,55 @@ int stp_accumulate_f32_f64(int64_t n, const float* x, double* y, void* stream) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ slice capture
+// RTMEngine::capture_single_slice (src/cpp/rtm/rtm_engine.cpp:83-165): one
+// plane of the forward and adjoint with-halo fields (strides ny_s * nz_s and
+// nz_s, the caller's) and of the image condition, non-finite values -> 0.
+// The image condition is indexed as the reference indexes it: by the two
+// in-plane coordinates only (axis X: ic[iy*nz + iz], Y: ic[ix*nz + iz], Z:
+// ic[ix*ny + iy]).
+namespace {
+__global__ void extract_slice_kernel(int axis, int64_t idx, int64_t nx, int64_t ny, int64_t nz, int64_t hw,
+                                     int64_t ny_s, int64_t nz_s, const double* fwd, const double* adj,
+                                     const double* ic, double* of, double* oa, double* oi) {
+    const int64_t d0 = axis == 0 ? ny : nx, d1 = axis == 2 ? ny : nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < d0 * d1;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t a = t / d1, b = t % d1;
+        int64_t src, ici;
+        if (axis == 0) {
+            src = (idx + hw) * (ny_s * nz_s) + (a + hw) * nz_s + (b + hw);
+            ici = a * nz + b;
+        } else if (axis == 1) {
+            src = (a + hw) * (ny_s * nz_s) + (idx + hw) * nz_s + (b + hw);
+            ici = a * nz + b;
+        } else {
+            src = (a + hw) * (ny_s * nz_s) + (b + hw) * nz_s + (idx + hw);
+            ici = a * ny + b;
+        }
+        const double f = fwd[src], g = adj[src], c = ic[ici];
+        of[t] = isfinite(f) ? f : 0.0;
+        oa[t] = isfinite(g) ? g : 0.0;
+        oi[t] = isfinite(c) ? c : 0.0;
+    }
+}
+}  // namespace
+
+extern "C" int stp_rtm_extract_slice_f64(int32_t axis, int64_t index, int64_t nx, int64_t ny, int64_t nz,
+                                         int32_t halo_width, int64_t ny_stride, int64_t nz_stride,
+                                         const double* fwd, const double* adj, const double* ic, double* out_fwd,
+                                         double* out_adj, double* out_ic, void* stream) {
+    if (axis < 0 || axis > 2 || nx < 1 || ny < 1 || nz < 1 || halo_width < 0 || !fwd || !adj || !ic || !out_fwd ||
+        !out_adj || !out_ic)
+        return stp::fail(STP_ERR_INVALID, "stp_rtm_extract_slice_f64: bad argument");
+    const int64_t ext = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    if (index < 0 || index >= ext) return stp::fail(STP_ERR_INVALID, "stp_rtm_extract_slice_f64: index out of range");
+    const int64_t n = (axis == 0 ? ny : nx) * (axis == 2 ? ny : nz);
+    int64_t nb = (n + 255) / 256;
+    nb = nb < 1 ? 1 : (nb > 65535 ? 65535 : nb);
+    hipLaunchKernelGGL(extract_slice_kernel, dim3((unsigned)nb), dim3(256), 0, stp::S(stream), (int)axis, index, nx, ny,
+                       nz, (int64_t)halo_width, ny_stride, nz_stride, fwd, adj, ic, out_fwd, out_adj, out_ic);
+    STP_HIP(hipGetLastError());
+    return STP_OK;
+}

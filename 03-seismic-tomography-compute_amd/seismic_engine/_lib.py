@@ -112,6 +112,8 @@ SIGNATURES: dict[str, list] = {
     "stp_terrain_free_surface_f64": [_i64, _i64, _i64, _i64, _vp, _vp],
     "stp_terrain_seafloor_f64": [_vp, _vp, _vp, _vp, _i64, _vp],
     "stp_terrain_absorbing_f64": [_i64, _i64, _i64, _i64, _i32, _f64, _vp, _vp],
+    "stp_rtm_extract_slice_f64": [_i32, _i64, _i64, _i64, _i64, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                  _vp],
 }
 _RESTYPES = {"stp_last_error": C.c_char_p}
 

@@ -1,8 +1,9 @@
 // RTM imaging condition (drop-in for the reference's src/cpp/rtm/rtm_engine.h:12-96).
 // Image and illumination stay on the GPU in FP64 and are copied to the host
-// when result() is read.  Slice capture for live viewing (extract_slices and
-// its callback) is not part of this build: specs and callback are stored, no
-// slice is produced.
+// when result() is read.  extract_slices gathers the slice planes of the
+// caller's host arrays on the host (a plane is too small to be worth a copy of
+// the volume to the device) and passes them to the slice callback; device
+// arrays go through stp_rtm_extract_slice_f64.
 #pragma once
 
 #include <atomic>
@@ -47,7 +48,13 @@ public:
     // fused reverse-time pass accumulates these)
     void accumulate_device_f32(const float* image_dev, const float* illum_dev);
 
-    void extract_slices(const double*, const double*, const double*, int64_t, int64_t, int64_t) {}
+    // rtm_engine.cpp:59-165: per slice spec (negative index = middle of the
+    // axis, VOLUME skipped), the plane of the with-halo forward / adjoint host
+    // arrays (y / z strides ny, nz) and of the image condition (the engine's
+    // image when null; indexed by the in-plane coordinates, as the reference
+    // does), non-finite values as 0, handed to the slice callback
+    void extract_slices(const double* forward_wavefield, const double* adjoint_wavefield,
+                        const double* image_condition, int64_t ny, int64_t nz, int64_t timestep);
 
     // reflectivity = image / illumination where illumination > 1e-10, else 0
     void finalize_image();
@@ -73,7 +80,7 @@ public:
     };
     using SliceCaptureCallback = std::function<void(const CapturedSlice&)>;
     void set_slice_callback(SliceCaptureCallback cb) { slice_cb_ = std::move(cb); }
-    int64_t total_captured_slices() const { return 0; }
+    int64_t total_captured_slices() const { return slices_captured_.load(std::memory_order_acquire); }
 
     // device arrays (interior-indexed FP64)
     const double* image_device() const { return image_dev_; }
@@ -90,6 +97,7 @@ private:
     mutable bool host_valid_ = false;
     std::vector<SliceSpec> slice_specs_;
     SliceCaptureCallback slice_cb_;
+    std::atomic<int64_t> slices_captured_{0};
 };
 
 }  // namespace seismic

@@ -354,6 +354,16 @@ int stp_depth_preconditioner_f64(int64_t nx, int64_t ny, int64_t nz, int64_t wat
 /* apply_preconditioning (:79-94): g *= weights */
 int stp_gradient_precondition_f64(int64_t n, const double* weights_dev, double* gradient_dev, void* stream);
 
+/* RTMEngine::capture_single_slice (src/cpp/rtm/rtm_engine.cpp:83-165): plane
+ * `index` along axis 0/1/2 (X/Y/Z) of the with-halo forward and adjoint
+ * fields (strides ny_stride*nz_stride, nz_stride) and of the image condition
+ * (indexed by the two in-plane coordinates, as the reference does), into
+ * dim0 x dim1 outputs (X: ny x nz, Y: nx x nz, Z: nx x ny); non-finite -> 0 */
+int stp_rtm_extract_slice_f64(int32_t axis, int64_t index, int64_t nx, int64_t ny, int64_t nz, int32_t halo_width,
+                              int64_t ny_stride, int64_t nz_stride, const double* fwd_dev, const double* adj_dev,
+                              const double* ic_dev, double* out_fwd_dev, double* out_adj_dev, double* out_ic_dev,
+                              void* stream);
+
 /* ---------------------------------------------------------------- terrain (bathymetry) boundary
  * TerrainBoundary of the reference's solver (src/cpp/solver/boundary.cpp),
  * FP64 with-halo x-major fields, bathymetry [nx_global][ny_global] (device

@@ -83,6 +83,31 @@ void ref_rtm_result(void* h, double* img, double* ill, double* refl) {
 }
 void ref_rtm_free(void* h) { delete static_cast<RTMEngine*>(h); }
 
+// RTMEngine::extract_slices with the given specs (axis, index pairs); the
+// captured slices are written one after another (forward, adjoint, ic per
+// slice) into out, their dims into dims (2 per slice).  Returns the count.
+int ref_rtm_extract(void* h, int nspec, const int* axes, const int64_t* idx, const double* fwd, const double* adj,
+                    const double* ic, int64_t nyh, int64_t nzh, int64_t step, double* out, int64_t* dims) {
+    auto* e = static_cast<RTMEngine*>(h);
+    std::vector<SliceSpec> specs;
+    for (int k = 0; k < nspec; ++k) specs.push_back({static_cast<SliceAxis>(axes[k]), idx[k], 1});
+    e->set_slice_specs(specs);
+    int count = 0;
+    int64_t off = 0;
+    e->set_slice_callback([&](const RTMEngine::CapturedSlice& s) {
+        const int64_t n = s.dim0 * s.dim1;
+        std::memcpy(out + off, s.forward_slice.data(), sizeof(double) * n);
+        std::memcpy(out + off + n, s.adjoint_slice.data(), sizeof(double) * n);
+        std::memcpy(out + off + 2 * n, s.ic_slice.data(), sizeof(double) * n);
+        off += 3 * n;
+        dims[2 * count] = s.dim0;
+        dims[2 * count + 1] = s.dim1;
+        ++count;
+    });
+    e->extract_slices(fwd, adj, ic, nyh, nzh, step);
+    return count;
+}
+
 // load_config: 21 numeric fields in a fixed order, plus the two strings.
 // Returns 0, or 1 when the reference threw (bad number).
 int ref_load_config(const char* path, double* out, char* misfit_type, char* output_dir, int cap) {

@@ -238,6 +238,18 @@ int cmd_rtm(const std::string& dir) {
     write(dir + "/image.f64", res.image);
     write(dir + "/illum.f64", res.illumination);
     write(dir + "/refl.f64", res.reflectivity);
+    // slice capture: X at 2, Y and Z at the middle, VOLUME skipped; ic = the image
+    eng.set_slice_specs({{SliceAxis::X, 2, 1}, {SliceAxis::Y, -1, 1}, {SliceAxis::VOLUME, 0, 1},
+                         {SliceAxis::Z, -1, 1}});
+    std::vector<double> slices;
+    eng.set_slice_callback([&](const RTMEngine::CapturedSlice& sl) {
+        slices.push_back((double)sl.index);
+        for (const auto* v : {&sl.forward_slice, &sl.adjoint_slice, &sl.ic_slice})
+            slices.insert(slices.end(), v->begin(), v->end());
+    });
+    eng.extract_slices(f.data(), a.data(), nullptr, ny + 2 * hw, nz + 2 * hw, 7);
+    write(dir + "/slices.f64", slices);
+    std::printf("slices_captured=%lld\n", (long long)eng.total_captured_slices());
 
     // run_rtm through the solver: source wavelet and receiver residuals
     GridParams gp{nx, ny, nz, 10.0, 10.0, 10.0, 0.0, 0.0, 0.0, hw};

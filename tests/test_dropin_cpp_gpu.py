@@ -146,6 +146,20 @@ def test_rtm_engine_and_run_rtm(tmp_path, cuda):
     assert np.allclose(f64(tmp_path, "image.f64"), img, rtol=1e-14, atol=1e-14)
     assert np.allclose(f64(tmp_path, "illum.f64"), ill, rtol=1e-14, atol=1e-14)
     assert np.allclose(f64(tmp_path, "refl.f64"), np.where(ill > 1e-10, img / np.where(ill > 0, ill, 1), 0))
+    # RTMEngine::extract_slices: index, forward plane, adjoint plane, image
+    # values per slice; rtm_engine.cpp:94-168 reads the image at the in-plane
+    # offset p * dim1 + q whatever the axis and index, i.e. its first
+    # dim0 * dim1 values
+    fh = np.where(np.isfinite(f), f, 0)
+    ah = np.where(np.isfinite(a), a, 0)
+    want = []
+    for ax, idx in ((0, 2), (1, n[1] // 2), (2, n[2] // 2)):
+        sel = [slice(hw, hw + k) for k in n]
+        sel[ax] = idx + hw
+        plane = fh[tuple(sel)].ravel()
+        want += [[idx], plane, ah[tuple(sel)].ravel(), img[:plane.size]]
+    assert np.array_equal(f64(tmp_path, "slices.f64"), np.concatenate(want))
+    assert "slices_captured=3" in text
     assert "rtm_disabled status=OK steps=0 ok=1" in text
     assert "rtm status=OK steps=60 ok=1" in text
     image = f64(tmp_path, "rtm_image.f64")
