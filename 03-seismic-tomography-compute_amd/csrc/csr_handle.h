@@ -19,5 +19,14 @@ struct stp_csr {
     double *dp = nullptr, *dap = nullptr, *dz = nullptr, *dr = nullptr, *ddinv = nullptr;
     stp::SolverScalars* sc = nullptr;
     size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0;
+    // solver layout (solver_layout.hip): columns are the cells of a
+    // gn[0]*gn[1]*gn[2] grid when gn[0] > 0; the renumbered copy is built on
+    // the first CGLS solve or by stp_csr_prepare_layout
+    int64_t gn[3] = {0, 0, 0};
+    bool lay_ready = false;
+    int *cperm = nullptr, *rsrc = nullptr;   // cell -> layout column; layout row -> row
+    int *lrp = nullptr, *lci = nullptr, *ltrp = nullptr, *ltci = nullptr;
+    float *lval = nullptr, *ltval = nullptr;
+    double *lx = nullptr, *lb = nullptr;
     size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;
 };

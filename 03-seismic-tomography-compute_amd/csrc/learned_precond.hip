@@ -218,43 +218,49 @@ __device__ __forceinline__ Cell cell_at(int64_t i, int n0, int n1, int n2) {
     return c;
 }
 
-// ((I + L_w) (s .* v))_i for the cell c
+// ((I + L_w) (s .* v))_i for the cell c; with PERM the value of cell j is
+// v[perm[j]]
+template <bool PERM = false>
 __device__ __forceinline__ float graph_apply(const Cell& c, const float* __restrict__ coef,
-                                             const float* __restrict__ v) {
-    const float yi = coef[c.i] * v[c.i];
+                                             const float* __restrict__ v, const int* __restrict__ perm = nullptr) {
+    auto at = [&](int64_t j) { return PERM ? v[perm[j]] : v[j]; };
+    const float yi = coef[c.i] * at(c.i);
     float u = yi;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const float wf = coef[(1 + d) * c.V + c.i];
         if (wf != 0.f) {
             const int64_t j = c.i + c.st[d];
-            u += wf * (yi - coef[j] * v[j]);
+            u += wf * (yi - coef[j] * at(j));
         }
         if (c.lo[d]) {
             const int64_t j = c.i - c.st[d];
-            u += coef[(1 + d) * c.V + j] * (yi - coef[j] * v[j]);
+            u += coef[(1 + d) * c.V + j] * (yi - coef[j] * at(j));
         }
     }
     return u;
 }
 
 // z = M r, optionally with the two FP64 dot partials of flexible CGLS
-// (z.r and z.(r - r_prev), one per block of 256 cells, like flex_dots_kernel)
+// (z.r and z.(r - r_prev), one per block of 256 cells, like flex_dots_kernel).
+// With PERM, r, zo and r_prev hold cell i's value at perm[i].
+template <bool PERM>
 __global__ void __launch_bounds__(256) apply_m_kernel(int n0, int n1, int n2, const float* __restrict__ coef,
                                                       const float* __restrict__ r, float* __restrict__ zo,
                                                       const int* done, const float* __restrict__ r_prev,
-                                                      double* part_zr, double* part_fl) {
+                                                      double* part_zr, double* part_fl, const int* __restrict__ perm) {
     if (done && *done) return;
     const int64_t V = (int64_t)n0 * n1 * n2;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double a = 0.0, b = 0.0;
     if (i < V) {
         const Cell c = cell_at(i, n0, n1, n2);
-        const float z = coef[i] * graph_apply(c, coef, r);
-        zo[i] = z;
+        const int64_t pi = PERM ? (int64_t)perm[i] : i;
+        const float z = coef[i] * graph_apply<PERM>(c, coef, r, perm);
+        zo[pi] = z;
         if (part_zr) {
-            a = (double)z * (double)r[i];
-            b = (double)z * ((double)r[i] - (double)r_prev[i]);
+            a = (double)z * (double)r[pi];
+            b = (double)z * ((double)r[pi] - (double)r_prev[pi]);
         }
     }
     if (!part_zr) return;
@@ -518,14 +524,18 @@ int net_backward(stp_lpc* h, hipStream_t st) {
 
 namespace stp {
 int lpc_apply(void* handle, const float* s, float* z, const int* done, const float* s_prev, double* part_zs,
-              double* part_fl, hipStream_t st) {
+              double* part_fl, hipStream_t st, const int* perm) {
     stp_lpc* h = static_cast<stp_lpc*>(handle);
     if (!h || !h->prepared) return fail(STP_ERR_STATE, "learned preconditioner: not prepared for a system");
     if ((part_zs || part_fl) && !(part_zs && part_fl && s_prev))
         return fail(STP_ERR_INVALID, "learned preconditioner: incomplete dot-product outputs");
     const int64_t V = h->V;
-    hipLaunchKernelGGL(apply_m_kernel, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, h->n0, h->n1, h->n2,
-                       h->coef, s, z, done, s_prev, part_zs, part_fl);
+    if (perm)
+        hipLaunchKernelGGL(apply_m_kernel<true>, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, h->n0, h->n1,
+                           h->n2, h->coef, s, z, done, s_prev, part_zs, part_fl, perm);
+    else
+        hipLaunchKernelGGL(apply_m_kernel<false>, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, h->n0, h->n1,
+                           h->n2, h->coef, s, z, done, s_prev, part_zs, part_fl, nullptr);
     STP_HIP(hipGetLastError());
     return STP_OK;
 }

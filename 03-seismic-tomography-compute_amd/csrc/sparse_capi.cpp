@@ -16,6 +16,7 @@
 #include "csr_handle.h"
 #include "learned_precond.h"
 #include "seismic_hip.h"
+#include "solver_layout.h"
 #include "sparse_kernels.h"
 
 using stp::fail;
@@ -43,6 +44,7 @@ static void csr_free(stp_csr* h) {
                   h->p, h->q, h->s, h->z, h->sp, h->dinv, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
     for (void* p : ps)
         if (p) hipFree(p);
+    stp::free_solver_layout(h);
 }
 
 static int finish_csr(stp_csr* h, hipStream_t st) {
@@ -95,6 +97,7 @@ int stp_csr_from_rays(int64_t nrays, const int64_t* n, double h_cell, const doub
     if (!h) return fail(STP_ERR_ALLOC, "stp_csr_from_rays: out of host memory");
     h->rows = (int)nrays;
     h->cols = (int)(n[0] * n[1] * n[2]);
+    for (int k = 0; k < 3; ++k) h->gn[k] = n[k];   // columns are grid cells: CGLS uses the solver layout
     int* counts = nullptr;
     hipError_t e = hipSuccess;
     int last_count = 0, last_off = 0;
@@ -240,6 +243,9 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
 // When the recursive residual reaches tol the true residual is recomputed in
 // FP64 (r = b - A x, s = A^T r - lam x); if it is still above tol the
 // iteration restarts from it (iterative refinement of the FP32 inner solve).
+// A matrix whose columns are grid cells (stp_csr_from_rays, stp_csr_set_grid)
+// is solved in its solver layout: b is gathered into layout row order, the
+// iteration runs on the renumbered copy and x is gathered back to cell order.
 int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t max_iter, int32_t precond,
              void* precond_handle, double* x, double* history, stp_solve_info* info, void* stream) {
     stp_csr* h = const_cast<stp_csr*>(hc);
@@ -250,6 +256,24 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         return fail(STP_ERR_STATE, "stp_cgls: learned preconditioner is not prepared for this matrix");
     hipStream_t st = S(stream);
     const int rows = h->rows, cols = h->cols;
+    const bool lay = h->gn[0] > 0;
+    if (lay) {
+        const int rc = stp::build_solver_layout(h, st);
+        if (rc != STP_OK) return rc;
+    }
+    const int* RP = lay ? h->lrp : h->rp;
+    const int* CI = lay ? h->lci : h->ci;
+    const float* VAL = lay ? h->lval : h->val;
+    const int* TRP = lay ? h->ltrp : h->trp;
+    const int* TCI = lay ? h->ltci : h->tci;
+    const float* TVAL = lay ? h->ltval : h->tval;
+    const int* perm = lay ? h->cperm : nullptr;
+    double* const x_out = x;
+    if (lay) {
+        STP_HIP(stp::gather_f64(rows, b, h->rsrc, h->lb, st));
+        b = h->lb;
+        x = h->lx;
+    }
     const int nbq = std::max(stp::spmv_blocks(rows, h->W), 1);
     const int nbs = std::max(stp::spmv_blocks(cols, h->WT), 1);
     const int nbc = std::max((cols + 255) / 256, 1);
@@ -270,18 +294,18 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     const bool flex = precond == 2;
     float* dinv = nullptr;
     if (precond == 1) {
-        STP_HIP(stp::launch_normal_diag(cols, h->trp, h->tval, lam, h->dinv, st));
+        STP_HIP(stp::launch_normal_diag(cols, TRP, TVAL, lam, h->dinv, st));
         dinv = h->dinv;
     }
     int* done = &h->sc->done;
 
     // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s
     auto start = [&](int set_s0) -> int {
-        STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z, h->part2,
+        STP_HIP(stp::normal_residual(h->WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
                                      h->part3, nullptr, st));
         if (flex) {
             STP_HIP(hipMemsetAsync(h->sp, 0, sizeof(float) * cols, st));
-            int rc = stp::lpc_apply(precond_handle, h->s, h->z, nullptr, h->sp, h->part4, h->part5, st);
+            int rc = stp::lpc_apply(precond_handle, h->s, h->z, nullptr, h->sp, h->part4, h->part5, st, perm);
             if (rc != STP_OK) return rc;
             STP_HIP(stp::launch_cgls_init(h->sc, h->part4, nbc, h->part3, nbs, set_s0, st));
         } else {
@@ -305,13 +329,13 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     std::vector<double> ps(nbs);
     for (;;) {
         for (; it < max_iter; ++it) {
-            STP_HIP(stp::spmv_f32(h->W, rows, h->rp, h->ci, h->val, h->p, h->q, h->part0, done, st));
+            STP_HIP(stp::spmv_f32(h->W, rows, RP, CI, VAL, h->p, h->q, h->part0, done, st));
             STP_HIP(stp::launch_cgls_alpha(h->sc, h->part0, nbq, h->part1, nbc, lam, st));
             STP_HIP(stp::launch_cgls_update(h->sc, cols, x, h->p, rows, h->r, h->q, st));
-            STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z,
+            STP_HIP(stp::normal_residual(h->WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z,
                                          h->part2, h->part3, done, st));
             if (flex) {
-                rc = stp::lpc_apply(precond_handle, h->s, h->z, done, h->sp, h->part4, h->part5, st);
+                rc = stp::lpc_apply(precond_handle, h->s, h->z, done, h->sp, h->part4, h->part5, st, perm);
                 if (rc != STP_OK) return rc;
                 STP_HIP(stp::launch_cgls_beta(h->sc, h->part4, nbc, h->part3, nbs, h->part5, nbc, tol, history, it,
                                               st));
@@ -329,8 +353,8 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         STP_HIP(hipStreamSynchronize(st));
         it = hs.iters;
         // true residual in FP64
-        STP_HIP(stp::residual(h->W, rows, h->rp, h->ci, h->val, x, b, h->r, st));
-        STP_HIP(stp::normal_residual(h->WT, cols, h->trp, h->tci, h->tval, h->r, x, lam, dinv, h->s, h->z, h->part2,
+        STP_HIP(stp::residual(h->W, rows, RP, CI, VAL, x, b, h->r, st));
+        STP_HIP(stp::normal_residual(h->WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
                                      h->part3, nullptr, st));
         STP_HIP(hipMemcpyAsync(ps.data(), h->part3, sizeof(double) * nbs, hipMemcpyDeviceToHost, st));
         STP_HIP(hipStreamSynchronize(st));
@@ -343,6 +367,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         if (rc != STP_OK) return rc;
         STP_HIP(hipMemsetAsync(&h->sc->done, 0, sizeof(int), st));
     }
+    if (lay) STP_HIP(stp::gather_f64(cols, x, h->cperm, x_out, st));
     if (info) {
         info->iterations = it;
         info->converged = (hs.done == 1 || hs.s0 == 0) && true_rel < tol;

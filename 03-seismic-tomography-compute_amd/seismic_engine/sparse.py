@@ -76,6 +76,16 @@ class DeviceCSR:
              stream_handle())
         return cls(hh)
 
+    def set_grid(self, n) -> None:
+        """Declare the columns as the cells of an n = (nx, ny, nz) grid (None
+        clears it): CGLS then runs in the Morton solver layout."""
+        nx, ny, nz = (0, 0, 0) if n is None else (int(k) for k in n)
+        call("stp_csr_set_grid", self._h, C.c_int64(nx), C.c_int64(ny), C.c_int64(nz))
+
+    def prepare_layout(self) -> None:
+        """Build the solver layout now instead of on the first CGLS solve."""
+        call("stp_csr_prepare_layout", self._h, stream_handle())
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):

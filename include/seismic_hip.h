@@ -174,6 +174,17 @@ int stp_csr_spmv(const stp_csr* A, int32_t transpose, const float* x_dev, float*
 /* same with FP64 x, FP64 accumulation and FP64 y (values stay FP32) */
 int stp_csr_spmv_f64(const stp_csr* A, int32_t transpose, const double* x_dev, double* y_dev, void* stream);
 
+/* Declare that the columns of A are the cells of an nx x ny x nz grid
+ * (column ix*ny*nz + iy*nz + iz); nx = ny = nz = 0 clears it.  stp_cgls then
+ * solves in a renumbered copy of A and A^T (cells in Morton order, rows
+ * sorted by the Morton code of their end cells) for gather locality, built on
+ * the first solve and kept with A (one more copy of A and A^T).  Results
+ * equal the plain solve up to FP32 summation order.  stp_csr_from_rays sets
+ * its grid.  New: the reference has no such layout. */
+int stp_csr_set_grid(stp_csr* A, int64_t nx, int64_t ny, int64_t nz);
+/* Build that layout now (no-op without a grid or when built).  Synchronises. */
+int stp_csr_prepare_layout(stp_csr* A, void* stream);
+
 /* SparseOps::conjugate_gradient / jacobi_preconditioned_cg on a square A:
  * x0 = 0, stop when |r|/|b| < tol (sparse_ops.cpp:39, 97-98), breakdown when
  * |p.Ap| < 1e-30; Jacobi = first diagonal entry with |a| > 1e-30, else 1.
