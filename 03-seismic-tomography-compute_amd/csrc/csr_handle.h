@@ -28,5 +28,6 @@ struct stp_csr {
     int *lrp = nullptr, *lci = nullptr, *ltrp = nullptr, *ltci = nullptr;
     float *lval = nullptr, *ltval = nullptr;
     double *lx = nullptr, *lb = nullptr;
+    int lW = 16, lWT = 8;   // lanes per row in the layout
     size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;
 };

@@ -195,6 +195,8 @@ int build_solver_layout(stp_csr* h, hipStream_t st) {
         if ((e = csr_transpose(rows, cols, nnz, h->lrp, h->lci, h->lval, h->ltrp, h->ltci, h->ltval, st)) !=
             hipSuccess)
             break;
+        h->lW = spmv_lanes_layout(rows ? (double)nnz / rows : 0.0, 0);
+        h->lWT = spmv_lanes_layout(cols ? (double)nnz / cols : 0.0, 1);
         e = hipStreamSynchronize(st);
     } while (0);
     hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(len);

@@ -268,14 +268,15 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     const int* TCI = lay ? h->ltci : h->tci;
     const float* TVAL = lay ? h->ltval : h->tval;
     const int* perm = lay ? h->cperm : nullptr;
+    const int W = lay ? h->lW : h->W, WT = lay ? h->lWT : h->WT;
     double* const x_out = x;
     if (lay) {
         STP_HIP(stp::gather_f64(rows, b, h->rsrc, h->lb, st));
         b = h->lb;
         x = h->lx;
     }
-    const int nbq = std::max(stp::spmv_blocks(rows, h->W), 1);
-    const int nbs = std::max(stp::spmv_blocks(cols, h->WT), 1);
+    const int nbq = std::max(stp::spmv_blocks(rows, W), 1);
+    const int nbs = std::max(stp::spmv_blocks(cols, WT), 1);
     const int nbc = std::max((cols + 255) / 256, 1);
     STP_HIP(grow(&h->r, &h->c_r, rows));
     STP_HIP(grow(&h->p, &h->c_p, cols));
@@ -301,7 +302,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
 
     // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s
     auto start = [&](int set_s0) -> int {
-        STP_HIP(stp::normal_residual(h->WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
+        STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
                                      h->part3, nullptr, st));
         if (flex) {
             STP_HIP(hipMemsetAsync(h->sp, 0, sizeof(float) * cols, st));
@@ -329,10 +330,10 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     std::vector<double> ps(nbs);
     for (;;) {
         for (; it < max_iter; ++it) {
-            STP_HIP(stp::spmv_f32(h->W, rows, RP, CI, VAL, h->p, h->q, h->part0, done, st));
+            STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, h->p, h->q, h->part0, done, st));
             STP_HIP(stp::launch_cgls_alpha(h->sc, h->part0, nbq, h->part1, nbc, lam, st));
             STP_HIP(stp::launch_cgls_update(h->sc, cols, x, h->p, rows, h->r, h->q, st));
-            STP_HIP(stp::normal_residual(h->WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z,
+            STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z,
                                          h->part2, h->part3, done, st));
             if (flex) {
                 rc = stp::lpc_apply(precond_handle, h->s, h->z, done, h->sp, h->part4, h->part5, st, perm);
@@ -353,8 +354,8 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         STP_HIP(hipStreamSynchronize(st));
         it = hs.iters;
         // true residual in FP64
-        STP_HIP(stp::residual(h->W, rows, RP, CI, VAL, x, b, h->r, st));
-        STP_HIP(stp::normal_residual(h->WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
+        STP_HIP(stp::residual(W, rows, RP, CI, VAL, x, b, h->r, st));
+        STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
                                      h->part3, nullptr, st));
         STP_HIP(hipMemcpyAsync(ps.data(), h->part3, sizeof(double) * nbs, hipMemcpyDeviceToHost, st));
         STP_HIP(hipStreamSynchronize(st));

@@ -12,6 +12,7 @@ struct SolverScalars {
 };
 
 int spmv_lanes(double mean_nnz, int transpose);
+int spmv_lanes_layout(double mean_nnz, int transpose);
 int spmv_blocks(int rows, int W);
 hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
                     double* part, const int* done, hipStream_t st);

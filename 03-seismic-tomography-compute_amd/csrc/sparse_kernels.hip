@@ -124,6 +124,21 @@ int spmv_lanes(double mean_nnz, int transpose) {
     return mean_nnz >= 96 ? 32 : (mean_nnz >= 24 ? 16 : 8);
 }
 
+// Lanes per row in the solver layout.  Consecutive rows there are neighbouring
+// cells (A^T) or neighbouring rays (A), so a wavefront that takes several
+// rows at once shares gathered lines between them: at 200k rays x 64^3 cells
+// A^T runs fastest with 8 lanes and A with 16 (a CGLS iteration 174 us,
+// against 237 us with the widths of the given order).  STP_SPMVL_W /
+// STP_SPMVLT_W override.
+int spmv_lanes_layout(double mean_nnz, int transpose) {
+    if (const char* ev = getenv(transpose ? "STP_SPMVLT_W" : "STP_SPMVL_W")) {
+        const int w = atoi(ev);
+        if (w == 8 || w == 16 || w == 32 || w == 64) return w;
+    }
+    if (transpose) return 8;
+    return mean_nnz >= 24 ? 16 : 8;
+}
+
 int spmv_blocks(int rows, int W) {
     const int nb = (rows + 256 / W - 1) / (256 / W);
     return nb < 1 ? 1 : (nb > MAX_BLOCKS ? MAX_BLOCKS : nb);
