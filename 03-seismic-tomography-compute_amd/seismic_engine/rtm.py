@@ -116,6 +116,118 @@ def decode_slice_frame(buf: bytes):
     return CapturedSlice(ts, SliceAxis(axis), index, d0, d1, arr(fo, fb), arr(ao, ab), arr(io, ib), stamp), seq
 
 
+# ------------------------------------------------------------------ mmap snapshot file
+# Version 2 of the reference's file (mmap_writer.h:28-39): the 120-byte
+# header (magic, header_size, file_size_bytes, frame_count, write_offset,
+# ring_start, ring_capacity, 8 reserved words) zero-padded to header_size =
+# 128, then [i64 size][frame] records from 128 to write_offset.  The
+# reference's C++ writer starts frames at 120 and its Python reader at 128;
+# here both sides use header_size.  A record that does not fit behind
+# write_offset restarts the ring at header_size (ring_start counts restarts)
+# and frame_count is the number of records between header_size and
+# write_offset.
+MMAP_FILE_MAGIC = 0x52544D534E4150
+MMAP_HEADER_BYTES = 128
+_FHDR = struct.Struct("<qqqqqqq")
+
+
+class MmapSnapshotWriter:
+    """MmapSnapshotWriter (mmap_writer.h:41-121) over Python's mmap."""
+
+    def __init__(self):
+        self._mm = None
+        self._fh = None
+        self.max_bytes = 0
+        self.frames_written = 0
+        self.frames_dropped = 0
+        self.bytes_written = 0
+        self.cycles = 0
+        self._seq = 0
+
+    def open(self, filename: str, max_bytes: int = 4 * 1024 ** 3) -> bool:
+        import mmap
+        self.close()
+        if max_bytes < MMAP_HEADER_BYTES + 8:
+            return False
+        self._fh = open(filename, "w+b")
+        self._fh.truncate(max_bytes)
+        self._mm = mmap.mmap(self._fh.fileno(), max_bytes)
+        self.max_bytes = int(max_bytes)
+        self._mm[:MMAP_HEADER_BYTES] = bytes(MMAP_HEADER_BYTES)
+        _FHDR.pack_into(self._mm, 0, MMAP_FILE_MAGIC, MMAP_HEADER_BYTES, self.max_bytes, 0, MMAP_HEADER_BYTES, 0, 0)
+        self.cycles += 1
+        return True
+
+    def is_open(self) -> bool:
+        return self._mm is not None
+
+    def _field(self, k: int) -> int:
+        return struct.unpack_from("<q", self._mm, 8 * k)[0]
+
+    def _set(self, k: int, v: int) -> None:
+        struct.pack_into("<q", self._mm, 8 * k, int(v))
+
+    def frame_count(self) -> int:
+        return self._field(3)
+
+    def write_offset(self) -> int:
+        return self._field(4)
+
+    def write_frame(self, frame: bytes) -> bool:
+        n = len(frame)
+        if self._mm is None or n <= 0 or n > self.max_bytes - MMAP_HEADER_BYTES - 8:
+            self.frames_dropped += 1
+            return False
+        off = self._field(4)
+        if off + 8 + n > self.max_bytes:
+            self._set(3, 0)
+            self._set(4, MMAP_HEADER_BYTES)
+            self._set(5, self._field(5) + 1)
+            self.cycles += 1
+            off = MMAP_HEADER_BYTES
+        struct.pack_into("<q", self._mm, off, n)
+        self._mm[off + 8:off + 8 + n] = frame
+        self._set(4, off + 8 + n)
+        self._set(3, self._field(3) + 1)
+        self.frames_written += 1
+        self.bytes_written += 8 + n
+        return True
+
+    def write_slice(self, sl: CapturedSlice) -> bool:
+        self._seq += 1
+        return self.write_frame(encode_slice_frame(sl, self._seq - 1))
+
+    def close(self) -> None:
+        if self._mm is not None:
+            self._mm.flush()
+            self._mm.close()
+            self._fh.close()
+        self._mm = None
+        self._fh = None
+
+
+def read_snapshot_file(filename: str) -> list[bytes]:
+    """The records of a snapshot file, oldest first (read_frame,
+    mmap_writer.cpp:230-290); [] when the magic or header is wrong."""
+    with open(filename, "rb") as f:
+        buf = f.read()
+    if len(buf) < MMAP_HEADER_BYTES:
+        return []
+    magic, hsize, _, _, end, _, _ = _FHDR.unpack_from(buf, 0)
+    if magic != MMAP_FILE_MAGIC or not 120 <= hsize <= len(buf):
+        return []
+    end = min(end, len(buf))
+    out, off = [], hsize
+    while off + 8 <= end:
+        n = struct.unpack_from("<q", buf, off)[0]
+        off += 8
+        if n <= 0 or n > end - off:
+            break
+        out.append(buf[off:off + n])
+        off += n
+    return out
+
+
 @dataclass
 class RTMImageCondition:
     image: np.ndarray

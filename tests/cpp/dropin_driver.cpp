@@ -25,6 +25,7 @@
 #include "core/mpi_context.h"
 #include "rtm/rtm_engine.h"
 #include "solver/boundary.h"
+#include "rtm/mmap_writer.h"
 #include "solver/wave_equation.h"
 #include "sparse/csr_matrix.h"
 #include "sparse/sparse_ops.h"
@@ -278,6 +279,60 @@ int cmd_rtm(const std::string& dir) {
     return 0;
 }
 
+// slice frames and the snapshot file, host only: nslices synthetic slices of
+// growing size written to snap.dat (max_bytes small enough to restart the
+// ring); every serialized frame also goes to frames.bin as [i64 n][frame]
+int cmd_mmap(const std::string& dir) {
+    const int n = (int)param(dir, "nslices");
+    const int64_t max_bytes = (int64_t)param(dir, "max_bytes");
+    MmapSnapshotWriter w;
+    if (!w.open(dir + "/snap.dat", max_bytes)) return 3;
+    std::vector<uint8_t> all, frame;
+    int ok = 0, roundtrip = 0;
+    for (int k = 0; k < n; ++k) {
+        RTMEngine::CapturedSlice s;
+        s.timestep = 10 * k;
+        s.axis = static_cast<SliceAxis>(k % 3);
+        s.index = k;
+        s.dim0 = 3 + k;
+        s.dim1 = 4 + (k % 2);
+        for (int64_t i = 0; i < s.dim0 * s.dim1; ++i) {
+            s.forward_slice.push_back(0.5 * k + i);
+            s.adjoint_slice.push_back(-1.25 * i + k);
+            s.ic_slice.push_back(1.0 / (1 + i + k));
+        }
+        s.capture_timestamp = 1000.0 + k;
+        serialize_slice(s, (uint64_t)k, frame);
+        const int64_t fs = (int64_t)frame.size();
+        all.insert(all.end(), reinterpret_cast<const uint8_t*>(&fs), reinterpret_cast<const uint8_t*>(&fs) + 8);
+        all.insert(all.end(), frame.begin(), frame.end());
+        ok += w.write_slice(s) ? 1 : 0;
+        RTMEngine::CapturedSlice back;
+        uint64_t seq = 0;
+        if (deserialize_slice(frame.data(), frame.size(), back, &seq) && seq == (uint64_t)k &&
+            back.forward_slice == s.forward_slice && back.adjoint_slice == s.adjoint_slice &&
+            back.ic_slice == s.ic_slice && back.dim0 == s.dim0 && back.axis == s.axis)
+            ++roundtrip;
+    }
+    frame[200] ^= 1;
+    RTMEngine::CapturedSlice bad;
+    const bool corrupt_rejected = !deserialize_slice(frame.data(), frame.size(), bad, nullptr);
+    const std::vector<uint8_t> huge((size_t)max_bytes, 0);
+    const bool huge_rejected = !w.write_frame(huge.data(), max_bytes);
+    const int64_t count = w.frame_count(), ring = w.header()->ring_start;
+    const MmapSnapshotWriter::Stats st = w.stats();
+    w.close();
+    std::vector<uint8_t> first;
+    const bool read0 = MmapSnapshotWriter::read_frame(dir + "/snap.dat", 0, first);
+    write(dir + "/frames.bin", all);
+    std::printf("written=%d roundtrip=%d corrupt_rejected=%d huge_rejected=%d frame_count=%lld ring_start=%lld "
+                "count_frames=%lld stats_written=%lld stats_dropped=%lld read0=%d read0_size=%zu\n",
+                ok, roundtrip, corrupt_rejected ? 1 : 0, huge_rejected ? 1 : 0, (long long)count, (long long)ring,
+                (long long)MmapSnapshotWriter::count_frames(dir + "/snap.dat"), (long long)st.frames_written,
+                (long long)st.frames_dropped_buffer_full, read0 ? 1 : 0, first.size());
+    return 0;
+}
+
 int cmd_health(const std::string& dir) {
     std::vector<double> d = read<double>(dir + "/field.f64");
     const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
@@ -347,6 +402,7 @@ int main(int argc, char** argv) {
         if (cmd == "rtm") return cmd_rtm(arg);
         if (cmd == "health") return cmd_health(arg);
         if (cmd == "terrain") return cmd_terrain(arg);
+        if (cmd == "mmap") return cmd_mmap(arg);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;
