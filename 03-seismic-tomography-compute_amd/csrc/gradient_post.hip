@@ -10,13 +10,16 @@
 //                   1 / (1 + (iz - water_depth_iz) * dz / 10000), :96-110
 //   preconditioning g <- g * p, :79-94
 // Arrays are x-major (index (ix*ny + iy)*nz + iz).  The max reduction is
-// exact (order-independent), so every result equals the reference's.
+// exact (order-independent), so every result equals the reference's.  The
+// volume kernels take z from the thread and y, x from the block grid
+// (launch3.h); the max is reduced in two stages without atomics.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
 
 #include "capi_common.h"
+#include "launch3.h"
 #include "seismic_hip.h"
 
 namespace {
@@ -30,47 +33,60 @@ unsigned blocks_for(int64_t n) {
 
 __global__ void tikhonov_kernel(int64_t nx, int64_t ny, int64_t nz, double lam, const double* __restrict__ v,
                                 double* __restrict__ g) {
-    const int64_t n = nx * ny * nz;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t iz = i % nz, iy = (i / nz) % ny, ix = i / (ny * nz);
+    const int64_t iz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iz >= nz) return;
+    const bool zin = iz > 0 && iz < nz - 1;
+    STP_FOR_PLANES_ROWS(ix, iy, nx, ny, stp::ROWS) {
+        const int64_t i = (ix * ny + iy) * nz + iz;
         const double c = v[i];
         double lap = 0.0;
         // same association as the reference: per axis (v+ - 2v) + v-, axes added x, y, z
         if (ix > 0 && ix < nx - 1) lap += v[i + ny * nz] - 2 * c + v[i - ny * nz];
         if (iy > 0 && iy < ny - 1) lap += v[i + nz] - 2 * c + v[i - nz];
-        if (iz > 0 && iz < nz - 1) lap += v[i + 1] - 2 * c + v[i - 1];
+        if (zin) lap += v[i + 1] - 2 * c + v[i - 1];
         g[i] = g[i] - lam * lap;
     }
 }
 
-__global__ void maxabs_kernel(int64_t n, const double* __restrict__ g, unsigned long long* out) {
-    double m = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double a = fabs(g[i]);
-        m = (a > m || a != a) ? a : m;   // NaN propagates, as numpy's max does
-    }
-    // non-negative doubles (and NaN, whose bit pattern with sign 0 is the
-    // largest) order like their unsigned bit patterns
-    for (int o = 32; o > 0; o >>= 1) {
-        const double p = __shfl_xor(m, o);
-        m = (p > m || p != p) ? p : m;
-    }
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+// NaN propagates, as numpy's max does
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// one partial max |g| per block; a final one-block pass reduces the partials
+__device__ __forceinline__ double block_max(double m) {
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) m = max_nan(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    return max_nan(max_nan(red[0], red[1]), max_nan(red[2], red[3]));
 }
 
-__global__ void scale_by_max_kernel(int64_t n, const unsigned long long* mx, double* g) {
-    const double m = __longlong_as_double((long long)*mx);
+__global__ void __launch_bounds__(256) maxabs_kernel(int64_t n, const double* __restrict__ g, double* part) {
+    double m = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = max_nan(m, fabs(g[i]));
+    m = block_max(m);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(256) max_final_kernel(int np, const double* part, double* mx) {
+    double m = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) m = max_nan(m, part[i]);
+    m = block_max(m);
+    if (threadIdx.x == 0) *mx = m;
+}
+
+__global__ void scale_by_max_kernel(int64_t n, const double* mx, double* g) {
+    const double m = *mx;
     if (!(m > 1e-30)) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         g[i] = g[i] / m;
 }
 
 __global__ void depth_weights_kernel(int64_t nx, int64_t ny, int64_t nz, int64_t wd, double dz, double* p) {
-    const int64_t n = nx * ny * nz;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t iz = i % nz;
-        p[i] = iz < wd ? 0.0 : 1.0 / (1.0 + (double)(iz - wd) * dz / 10000.0);
-    }
+    const int64_t iz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iz >= nz) return;
+    const double w = iz < wd ? 0.0 : 1.0 / (1.0 + (double)(iz - wd) * dz / 10000.0);
+    STP_FOR_PLANES_ROWS(ix, iy, nx, ny, stp::ROWS) p[(ix * ny + iy) * nz + iz] = w;
 }
 
 __global__ void multiply_kernel(int64_t n, const double* __restrict__ p, double* __restrict__ g) {
@@ -90,8 +106,7 @@ int stp_gradient_tikhonov_f64(int64_t nx, int64_t ny, int64_t nz, double lambda,
                               double* gradient, void* stream) {
     if (!dims_ok(nx, ny, nz) || !velocity || !gradient)
         return stp::fail(STP_ERR_INVALID, "stp_gradient_tikhonov_f64: bad argument");
-    const int64_t n = nx * ny * nz;
-    hipLaunchKernelGGL(tikhonov_kernel, dim3(blocks_for(n)), dim3(256), 0, stp::S(stream), nx, ny, nz, lambda,
+    hipLaunchKernelGGL(tikhonov_kernel, stp::grid3(nx, ny, nz), dim3(256), 0, stp::S(stream), nx, ny, nz, lambda,
                        velocity, gradient);
     STP_HIP(hipGetLastError());
     return STP_OK;
@@ -102,37 +117,34 @@ int stp_gradient_normalize_f64(int64_t n, double* gradient, double* max_abs_host
     if (max_abs_host) *max_abs_host = 0.0;
     if (n == 0) return STP_OK;
     hipStream_t st = stp::S(stream);
-    unsigned long long* mx = nullptr;
-    STP_HIP(hipMallocAsync(reinterpret_cast<void**>(&mx), sizeof(unsigned long long), st));
-    hipError_t e = hipMemsetAsync(mx, 0, sizeof(unsigned long long), st);
+    const int64_t nb64 = (n + 255) / 256;
+    const int np = (int)(nb64 < 2048 ? nb64 : 2048);
+    double* buf = nullptr;   // np partials, then the max
+    STP_HIP(hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(double) * (size_t)(np + 1), st));
+    hipLaunchKernelGGL(maxabs_kernel, dim3(np), dim3(256), 0, st, n, gradient, buf);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(maxabs_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, gradient, mx);
+        hipLaunchKernelGGL(max_final_kernel, dim3(1), dim3(256), 0, st, np, buf, buf + np);
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(scale_by_max_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, mx, gradient);
+        hipLaunchKernelGGL(scale_by_max_kernel, dim3(blocks_for(n)), dim3(256), 0, st, n, buf + np, gradient);
         e = hipGetLastError();
     }
-    unsigned long long bits = 0;
-    if (e == hipSuccess && max_abs_host) e = hipMemcpyAsync(&bits, mx, sizeof bits, hipMemcpyDeviceToHost, st);
-    hipError_t e2 = hipFreeAsync(mx, st);
+    double m = 0.0;
+    if (e == hipSuccess && max_abs_host) e = hipMemcpyAsync(&m, buf + np, sizeof m, hipMemcpyDeviceToHost, st);
+    hipError_t e2 = hipFreeAsync(buf, st);
     if (e == hipSuccess && max_abs_host) e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = e2;
     if (e != hipSuccess) return stp::hip_fail(e, "stp_gradient_normalize_f64");
-    if (max_abs_host) {
-        double m;
-        static_assert(sizeof m == sizeof bits, "size");
-        __builtin_memcpy(&m, &bits, sizeof m);
-        *max_abs_host = m;
-    }
+    if (max_abs_host) *max_abs_host = m;
     return STP_OK;
 }
 
 int stp_depth_preconditioner_f64(int64_t nx, int64_t ny, int64_t nz, int64_t water_depth_iz, double dz,
                                  double* out, void* stream) {
     if (!dims_ok(nx, ny, nz) || !out) return stp::fail(STP_ERR_INVALID, "stp_depth_preconditioner_f64: bad argument");
-    const int64_t n = nx * ny * nz;
-    hipLaunchKernelGGL(depth_weights_kernel, dim3(blocks_for(n)), dim3(256), 0, stp::S(stream), nx, ny, nz,
+    hipLaunchKernelGGL(depth_weights_kernel, stp::grid3(nx, ny, nz), dim3(256), 0, stp::S(stream), nx, ny, nz,
                        water_depth_iz, dz, out);
     STP_HIP(hipGetLastError());
     return STP_OK;

@@ -29,6 +29,7 @@
 #include <cstdint>
 
 #include "capi_common.h"
+#include "launch3.h"
 #include "seismic_hip.h"
 
 namespace {
@@ -65,33 +66,37 @@ __global__ void bathy_kernel(int64_t nx, int64_t ny, double dx, double dy, doubl
     }
 }
 
+// interior cells; launched over the interior box (launch3.h)
 __global__ void velocity_mask_kernel(Column c, const double* bathy, double water_vp, double sediment_vp, double* v) {
     const int64_t nxi = c.nxh - 2 * c.hw, nyi = c.nyh - 2 * c.hw, nzi = c.nzh - 2 * c.hw;
-    const int64_t n = nxi * nyi * nzi;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ix = t / (nyi * nzi) + c.hw, iy = (t / nzi) % nyi + c.hw, iz = t % nzi + c.hw;
-        const int64_t sf = (int64_t)(depth_at(c, bathy, ix - c.hw + c.ox, iy - c.hw + c.oy) / c.dz);
-        const int64_t idx = (ix * c.nyh + iy) * c.nzh + iz;
-        if (iz - c.hw < sf) {
-            v[idx] = water_vp;
-        } else {
-            const double grad = 1.0 + 0.3 * ((iz - c.hw - sf) * c.dz / 5000.0);
-            v[idx] = fmin(sediment_vp * grad, 8500.0);
+    const int64_t tz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tz >= nzi) return;
+    const int64_t iz = tz + c.hw;
+    STP_FOR_PLANES_ROWS(tx, ty, nxi, nyi, stp::ROWS) {
+            const int64_t ix = tx + c.hw, iy = ty + c.hw;
+            const int64_t sf = (int64_t)(depth_at(c, bathy, tx + c.ox, ty + c.oy) / c.dz);
+            const int64_t idx = (ix * c.nyh + iy) * c.nzh + iz;
+            if (iz - c.hw < sf) {
+                v[idx] = water_vp;
+            } else {
+                const double grad = 1.0 + 0.3 * ((iz - c.hw - sf) * c.dz / 5000.0);
+                v[idx] = fmin(sediment_vp * grad, 8500.0);
+            }
         }
-    }
 }
 
+// every cell of the with-halo box (launch3.h)
 __global__ void boundary_mask_kernel(Column c, const double* bathy, uint8_t* mask) {
-    const int64_t n = c.nxh * c.nyh * c.nzh;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ix = i / (c.nyh * c.nzh), iy = (i / c.nzh) % c.nyh, iz = i % c.nzh;
-        uint8_t m = 0;
-        if (ix >= c.hw && ix < c.nxh - c.hw && iy >= c.hw && iy < c.nyh - c.hw && iz >= c.hw && iz < c.nzh - c.hw) {
-            const double zpos = c.origin_z + (iz - c.hw + c.oz) * c.dz;
-            m = zpos >= depth_at(c, bathy, ix - c.hw + c.ox, iy - c.hw + c.oy) ? 1 : 2;
+    const int64_t iz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iz >= c.nzh) return;
+    const bool zin = iz >= c.hw && iz < c.nzh - c.hw;
+    const double zpos = c.origin_z + (iz - c.hw + c.oz) * c.dz;
+    STP_FOR_PLANES_ROWS(ix, iy, c.nxh, c.nyh, stp::ROWS) {
+            uint8_t m = 0;
+            if (zin && ix >= c.hw && ix < c.nxh - c.hw && iy >= c.hw && iy < c.nyh - c.hw)
+                m = zpos >= depth_at(c, bathy, ix - c.hw + c.ox, iy - c.hw + c.oy) ? 1 : 2;
+            mask[(ix * c.nyh + iy) * c.nzh + iz] = m;
         }
-        mask[i] = m;
-    }
 }
 
 __global__ void free_surface_kernel(int64_t nxh, int64_t nyh, int64_t nzh, int64_t hw, double* f) {
@@ -118,24 +123,27 @@ __global__ void seafloor_kernel(Column c, const double* bathy, int64_t size, dou
     }
 }
 
+// interior cells; launched over the interior box (launch3.h)
 __global__ void absorbing_kernel(int64_t nxh, int64_t nyh, int64_t nzh, int64_t hw, int sw, double damping,
                                  double* f) {
     const int64_t nxi = nxh - 2 * hw, nyi = nyh - 2 * hw, nzi = nzh - 2 * hw;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nxi * nyi * nzi;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ix = t / (nyi * nzi) + hw, iy = (t / nzi) % nyi + hw, iz = t % nzi + hw;
-        double d = (double)(iz - hw);
-        d = fmin(d, (double)(nzh - hw - 1 - iz));
-        d = fmin(d, (double)(ix - hw));
-        d = fmin(d, (double)(nxh - hw - 1 - ix));
-        d = fmin(d, (double)(iy - hw));
-        d = fmin(d, (double)(nyh - hw - 1 - iy));
-        if (d < sw && d >= 0) {
-            const double factor = 1.0 - damping * (sw - d) * (sw - d) / (double)(sw * sw);
-            const int64_t idx = (ix * nyh + iy) * nzh + iz;
-            f[idx] *= factor;
+    const int64_t tz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tz >= nzi) return;
+    const int64_t iz = tz + hw;
+    STP_FOR_PLANES_ROWS(tx, ty, nxi, nyi, stp::ROWS) {
+            const int64_t ix = tx + hw, iy = ty + hw;
+            double d = (double)(iz - hw);
+            d = fmin(d, (double)(nzh - hw - 1 - iz));
+            d = fmin(d, (double)(ix - hw));
+            d = fmin(d, (double)(nxh - hw - 1 - ix));
+            d = fmin(d, (double)(iy - hw));
+            d = fmin(d, (double)(nyh - hw - 1 - iy));
+            if (d < sw && d >= 0) {
+                const double factor = 1.0 - damping * (sw - d) * (sw - d) / (double)(sw * sw);
+                const int64_t idx = (ix * nyh + iy) * nzh + iz;
+                f[idx] *= factor;
+            }
         }
-    }
 }
 
 bool column_of(const int64_t* ext, const double* geo, Column* c) {
@@ -167,9 +175,9 @@ int stp_terrain_velocity_mask_f64(const int64_t* ext, const double* geo, const d
     Column c;
     if (!column_of(ext, geo, &c) || !bathy || !velocity)
         return stp::fail(STP_ERR_INVALID, "stp_terrain_velocity_mask_f64: bad argument");
-    const int64_t n = (c.nxh - 2 * c.hw) * (c.nyh - 2 * c.hw) * (c.nzh - 2 * c.hw);
-    hipLaunchKernelGGL(velocity_mask_kernel, dim3(blocks_for(n)), dim3(256), 0, stp::S(stream), c, bathy, water_vp,
-                       sediment_vp, velocity);
+    const dim3 g3 = stp::grid3(c.nxh - 2 * c.hw, c.nyh - 2 * c.hw, c.nzh - 2 * c.hw);
+    hipLaunchKernelGGL(velocity_mask_kernel, g3, dim3(256), 0, stp::S(stream), c, bathy, water_vp, sediment_vp,
+                       velocity);
     STP_HIP(hipGetLastError());
     return STP_OK;
 }
@@ -179,7 +187,7 @@ int stp_terrain_boundary_mask(const int64_t* ext, const double* geo, const doubl
     Column c;
     if (!column_of(ext, geo, &c) || !bathy || !mask)
         return stp::fail(STP_ERR_INVALID, "stp_terrain_boundary_mask: bad argument");
-    hipLaunchKernelGGL(boundary_mask_kernel, dim3(blocks_for(c.nxh * c.nyh * c.nzh)), dim3(256), 0, stp::S(stream), c,
+    hipLaunchKernelGGL(boundary_mask_kernel, stp::grid3(c.nxh, c.nyh, c.nzh), dim3(256), 0, stp::S(stream), c,
                        bathy, mask);
     STP_HIP(hipGetLastError());
     return STP_OK;
@@ -209,7 +217,7 @@ int stp_terrain_absorbing_f64(int64_t nxh, int64_t nyh, int64_t nzh, int64_t hw,
                               double damping, double* field, void* stream) {
     if (hw < 0 || nxh <= 2 * hw || nyh <= 2 * hw || nzh <= 2 * hw || !field)
         return stp::fail(STP_ERR_INVALID, "stp_terrain_absorbing_f64: bad argument");
-    hipLaunchKernelGGL(absorbing_kernel, dim3(blocks_for((nxh - 2 * hw) * (nyh - 2 * hw) * (nzh - 2 * hw))), dim3(256),
+    hipLaunchKernelGGL(absorbing_kernel, stp::grid3(nxh - 2 * hw, nyh - 2 * hw, nzh - 2 * hw), dim3(256),
                        0, stp::S(stream), nxh, nyh, nzh, hw, (int)sponge_width, damping, field);
     STP_HIP(hipGetLastError());
     return STP_OK;

@@ -304,6 +304,93 @@ def learned_heldout(dev, n: int = 96, lam: float = 1e-2, tol: float = 1e-6, net=
     return out
 
 
+def aux_workload(dev, n: int = 512, cpu_n: int = 128, hw: int = 4, reps: int = 10):
+    """Per-call rate of the inversion loop's gradient post-processing, the
+    terrain boundary and RTM slice extraction at n^3, FP64: HIP events on the
+    launching stream around `reps` calls; algorithmic bytes per call (each
+    array read or written once) over that time, against 8 TB/s.  CPU: the
+    numpy restatements of the post-processing (oracle/gradient_post.py) on a
+    cpu_n^3 sample, scaled per cell."""
+    import ctypes as C
+    from types import SimpleNamespace
+
+    import numpy as np
+    import torch
+    from oracle import gradient_post as ogp
+    from seismic_engine._lib import call, ptr, stream_handle
+    from seismic_engine.sparse import SparseOps
+    from seismic_engine.terrain import TerrainBoundary
+    V = n ** 3
+    n3 = (C.c_int64(n), C.c_int64(n), C.c_int64(n))
+    gen = torch.Generator(device=dev).manual_seed(7)
+    g = torch.randn(V, dtype=torch.float64, device=dev, generator=gen)
+    v = 1500.0 + 3000.0 * torch.rand(V, dtype=torch.float64, device=dev, generator=gen)
+    w = torch.empty_like(g)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    out = {"grid": [n, n, n], "dtype": "f64"}
+
+    def rec(name, sec, nbytes, what):
+        out[name] = {"us": round(sec * 1e6, 1), "bytes_per_call": int(nbytes), "gb_s": round(nbytes / sec / 1e9, 1),
+                     "frac_hbm": round(nbytes / sec / 1e9 / HBM_PEAK_GBS, 4), "bytes": what}
+
+    st = stream_handle
+    rec("tikhonov", timed(lambda: call("stp_gradient_tikhonov_f64", *n3, C.c_double(1e-4), ptr(v), ptr(g), st())),
+        24 * V, "v, g read, g written")
+    rec("normalize", timed(lambda: call("stp_gradient_normalize_f64", C.c_int64(V), ptr(g), None, st())),
+        24 * V, "max pass reads g; scale pass reads and writes g")
+    rec("depth_preconditioner", timed(lambda: call("stp_depth_preconditioner_f64", *n3, C.c_int64(16),
+                                                   C.c_double(200.0), ptr(w), st())), 8 * V, "weights written")
+    rec("precondition", timed(lambda: call("stp_gradient_precondition_f64", C.c_int64(V), ptr(w), ptr(g), st())),
+        24 * V, "w, g read, g written")
+    rec("smoothing_sigma2", timed(lambda: SparseOps.smooth_gradient(g, n, n, n, 2.0)), 48 * V,
+        "three 1-D passes, each reads and writes the volume")
+    grid = SimpleNamespace(nx_global=n, ny_global=n, nz_global=n, dx=10.0, dy=10.0, dz=10.0, origin_x=0.0,
+                           origin_y=0.0, origin_z=0.0, halo_width=hw)
+    tb = TerrainBoundary(grid, 1500.0)
+    tb.generate_synthetic_bathymetry(0.05, 20.0)
+    del v, w
+    torch.cuda.empty_cache()
+    nh = n + 2 * hw
+    vh = torch.zeros(nh ** 3, dtype=torch.float64, device=dev)
+    rec("terrain_velocity_mask", timed(lambda: tb.update_velocity_mask(vh, 1500.0, 1800.0)), 8 * V,
+        "interior velocity written")
+    rec("terrain_boundary_mask", timed(lambda: tb.build_boundary_mask()), nh ** 3, "one byte per cell written")
+    f = torch.randn(nh ** 3, dtype=torch.float64, device=dev, generator=gen)
+    ic = g
+    o = torch.empty(3, n * n, dtype=torch.float64, device=dev)
+    for axis, name in ((0, "slice_x"), (2, "slice_z")):
+        rec(name, timed(lambda: call("stp_rtm_extract_slice_f64", axis, C.c_int64(n // 2), *n3, hw, C.c_int64(nh),
+                                     C.c_int64(nh), ptr(f), ptr(vh), ptr(ic), ptr(o[0]), ptr(o[1]), ptr(o[2]), st())),
+            48 * n * n, "three planes read and written")
+    del f, vh, g, o, tb
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(7)
+    gc = rng.standard_normal(cpu_n ** 3)
+    vc = 1500.0 + 3000.0 * rng.random(cpu_n ** 3)
+    scale = V / cpu_n ** 3
+    cpu = {}
+    for name, fn in (("tikhonov", lambda: ogp.tikhonov(vc, gc, cpu_n, cpu_n, cpu_n, 1e-4)),
+                     ("normalize", lambda: ogp.normalize(gc)),
+                     ("depth_preconditioner", lambda: ogp.depth_weights(cpu_n, cpu_n, cpu_n, 16))):
+        t0 = time.perf_counter()
+        fn()
+        cpu[name + "_ms_scaled"] = round((time.perf_counter() - t0) * scale * 1e3, 1)
+    out["cpu_baseline"] = dict(cpu, sample=f"numpy restatements (oracle/gradient_post.py) at {cpu_n}^3, 1 process, "
+                                           f"scaled by {scale:.0f} to {n}^3")
+    return out
+
+
 def traveltime_workload(dev, n: int = 64, npicks: int = 200_000, h: float = 10.0, cpu_sample: int = 2000):
     """Travel-time tomography (the reference's straight-ray sampled operator):
     npicks source/station pairs on opposite faces of an n^3 model of h-metre
@@ -375,6 +462,10 @@ def main() -> int:
                          "sponge on these 1500-4500 m/s models; the kernel's memory traffic is the same)")
     ap.add_argument("--no-grad", action="store_true", help="skip the 256^3 x 16-shot adjoint-gradient timing")
     ap.add_argument("--no-tt", action="store_true", help="skip the travel-time tomography timing")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the gradient post-processing, terrain and slice-extraction timing")
+    ap.add_argument("--aux-only", action="store_true", help="only the gradient post-processing, terrain and "
+                                                            "slice-extraction timing (one JSON object)")
     ap.add_argument("--cg-only", action="store_true",
                     help="only the tomography-solve timing (one JSON object, for profiling the CGLS kernels)")
     ap.add_argument("--cg-kinds", default="jacobi,none,learned",
@@ -403,6 +494,9 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes_read": 4 << 28, "bytes_written": 4 << 28, "copies": 3}))
+        return 0
+    if args.aux_only:
+        print(json.dumps({"aux": aux_workload(dev)}), flush=True)
         return 0
     if args.cg_only:
         print(json.dumps({"cg": cg_workload(dev, kinds=tuple(args.cg_kinds.split(",")),
@@ -501,6 +595,9 @@ def main() -> int:
     if not args.no_tt:
         log("travel-time tomography: 200k picks, 64^3")
         result["traveltime"] = traveltime_workload(dev)
+    if not args.no_aux:
+        log("gradient post-processing, terrain boundary and slice extraction at 512^3 (FP64)")
+        result["aux"] = aux_workload(dev)
     if not args.no_cg:
         log("tomography solve: 200k rays, 64^3, CGLS to 1e-6")
         result["cg"] = cg_workload(dev)
