@@ -71,6 +71,111 @@ __global__ void __launch_bounds__(256) smooth_pass_kernel(int64_t nx, int64_t ny
         }
 }
 
+// Fast forms for r <= RT, same taps added in the same order (same bits):
+//
+// x / y pass (AX = 0 / 1): each thread walks SEG outputs of one line along
+// the axis with the 2 RT + 1 taps of the current output in registers plus PF
+// values loaded ahead, so an output costs one load and the loads of PF later
+// outputs are in flight.  Threads along z, blocks over the other coordinate
+// (grid y) and over segments (grid z).
+template <int AX, int RT, int SEG, int PF>
+__global__ void __launch_bounds__(256) smooth_walk_kernel(int64_t nx, int64_t ny, int64_t nz, int r, Weights wt,
+                                                           const double* __restrict__ in, double* __restrict__ out) {
+    constexpr int NW = 2 * RT + 1 + PF;
+    __shared__ double ws[MAXR + 1];
+    for (int k = threadIdx.x; k <= r; k += blockDim.x) ws[k] = wt.w[k];
+    __syncthreads();
+    const int64_t iz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (iz >= nz) return;
+    const int64_t len = AX == 0 ? nx : ny, nother = AX == 0 ? ny : nx;
+    const int64_t stride = AX == 0 ? ny * nz : nz;
+    for (int64_t o = blockIdx.y; o < nother; o += gridDim.y) {
+        const int64_t i0 = (AX == 0 ? o * nz : o * ny * nz) + iz;   // the line's element at axis coordinate 0
+        auto ld = [&](int64_t j) {
+            const bool ok = j >= 0 && j < len;
+            const double x = in[i0 + (ok ? j : 0) * stride];   // in bounds either way
+            return ok ? x : 0.0;
+        };
+        for (int64_t c0 = (int64_t)blockIdx.z * SEG; c0 < len; c0 += (int64_t)gridDim.z * SEG) {
+            const int64_t c1 = c0 + SEG < len ? c0 + SEG : len;
+            double win[NW];   // win[t] = value at c - RT + t
+#pragma unroll
+            for (int t = 0; t < NW; ++t) win[t] = ld(c0 - RT + t);
+            for (int64_t c = c0; c < c1; ++c) {
+                double s = 0.0, ws_sum = 0.0;
+#pragma unroll
+                for (int t = 0; t < 2 * RT + 1; ++t) {
+                    const int d = t - RT;
+                    if (d >= -r && d <= r && c + d >= 0 && c + d < len) {
+                        const double wd = ws[t < RT ? RT - t : t - RT];
+                        s += wd * win[t];
+                        ws_sum += wd;
+                    }
+                }
+                out[i0 + c * stride] = s / ws_sum;
+                const double nxt = ld(c + RT + PF + 1);
+#pragma unroll
+                for (int t = 0; t < NW - 1; ++t) win[t] = win[t + 1];
+                win[NW - 1] = nxt;
+            }
+        }
+    }
+}
+
+// z pass: a block stages TZ outputs of one row plus RT halo values on each
+// side in LDS and takes the taps from there; blocks walk `rows` rows.
+template <int RT, int ROWS_>
+__global__ void __launch_bounds__(256) smooth_z_lds_kernel(int64_t nx, int64_t ny, int64_t nz, int r, Weights wt,
+                                                            const double* __restrict__ in, double* __restrict__ out) {
+    constexpr int TZ = 256;
+    __shared__ double ws[MAXR + 1];
+    __shared__ double tile[TZ + 2 * RT];
+    for (int k = threadIdx.x; k <= r; k += blockDim.x) ws[k] = wt.w[k];
+    const int64_t z0 = (int64_t)blockIdx.x * TZ, iz = z0 + threadIdx.x;
+    for (int64_t ix = blockIdx.z; ix < nx; ix += gridDim.z)
+        for (int64_t y0 = (int64_t)blockIdx.y * ROWS_; y0 < ny; y0 += (int64_t)gridDim.y * ROWS_)
+            for (int64_t iy = y0, y1 = (y0 + ROWS_ < ny ? y0 + ROWS_ : ny); iy < y1; ++iy) {
+                const int64_t row = (ix * ny + iy) * nz;
+                __syncthreads();   // the previous row's taps are read
+                for (int k = threadIdx.x; k < TZ + 2 * RT; k += blockDim.x) {
+                    const int64_t z = z0 - RT + k;
+                    tile[k] = (z >= 0 && z < nz) ? in[row + z] : 0.0;
+                }
+                __syncthreads();
+                if (iz < nz) {
+                    double s = 0.0, ws_sum = 0.0;
+#pragma unroll
+                    for (int t = 0; t < 2 * RT + 1; ++t) {
+                        const int d = t - RT;
+                        if (d >= -r && d <= r && iz + d >= 0 && iz + d < nz) {
+                            const double wd = ws[t < RT ? RT - t : t - RT];
+                            s += wd * tile[threadIdx.x + t];
+                            ws_sum += wd;
+                        }
+                    }
+                    out[row + iz] = s / ws_sum;
+                }
+            }
+}
+
+inline dim3 walk_grid(int64_t nz, int64_t nother, int64_t len, int seg) {
+    const int64_t bz = (nz + 255) / 256, nseg = (len + seg - 1) / seg;
+    return dim3((unsigned)(bz < 1 ? 1 : bz), (unsigned)(nother < 1 ? 1 : (nother > 65535 ? 65535 : nother)),
+                (unsigned)(nseg < 1 ? 1 : (nseg > 65535 ? 65535 : nseg)));
+}
+
+hipError_t smooth_passes_fast(int64_t nx, int64_t ny, int64_t nz, int r, const Weights& wt, const double* in,
+                              double* out, double* work, hipStream_t st) {
+    constexpr int RT = 8, SEG = 64, PF = 4;
+    hipLaunchKernelGGL((smooth_z_lds_kernel<RT, ROWS>), grid3(nx, ny, nz), dim3(256), 0, st, nx, ny, nz, r, wt, in,
+                       out);
+    hipLaunchKernelGGL((smooth_walk_kernel<1, RT, SEG, PF>), walk_grid(nz, nx, ny, SEG), dim3(256), 0, st, nx, ny, nz,
+                       r, wt, out, work);
+    hipLaunchKernelGGL((smooth_walk_kernel<0, RT, SEG, PF>), walk_grid(nz, ny, nx, SEG), dim3(256), 0, st, nx, ny, nz,
+                       r, wt, work, out);
+    return hipGetLastError();
+}
+
 template <int RT>
 hipError_t smooth_passes(int64_t nx, int64_t ny, int64_t nz, int r, const Weights& wt, const double* in,
                          double* out, double* work, hipStream_t st) {
@@ -104,7 +209,7 @@ int stp_smooth_gradient_f64(int64_t nx, int64_t ny, int64_t nz, double sigma, co
     stp::Weights wt{};
     for (int d = 0; d <= r; ++d) wt.w[d] = std::exp(-(double)(d * d) / (2.0 * sigma * sigma));
     hipStream_t st = S(stream);
-    STP_HIP(r <= 8 ? stp::smooth_passes<8>(nx, ny, nz, r, wt, in, out, work, st)
+    STP_HIP(r <= 8 ? stp::smooth_passes_fast(nx, ny, nz, r, wt, in, out, work, st)
                    : stp::smooth_passes<0>(nx, ny, nz, r, wt, in, out, work, st));
     return STP_OK;
 }

@@ -35,10 +35,13 @@ def test_health_functions_follow_inversion_call_pattern(cuda):
     assert t.cpu().tolist() == [1e10, -1e10, 0.0, 3.0]
 
 
-@pytest.mark.parametrize("sigma", [0.5, 1.0, 2.0, 2.7])
-def test_smooth_gradient_matches_oracle(sigma, cuda):
+# radius int(3 sigma): <= 8 takes the windowed/LDS passes, 10 the general one;
+# the larger boxes cross the 64-cell walk segments and the 256-cell z tiles
+@pytest.mark.parametrize("sigma,n", [(0.5, (12, 10, 9)), (1.0, (12, 10, 9)), (2.0, (12, 10, 9)),
+                                     (2.7, (12, 10, 9)), (3.5, (12, 10, 9)), (1.0, (70, 40, 300)),
+                                     (2.7, (66, 8, 258))])
+def test_smooth_gradient_matches_oracle(sigma, n, cuda):
     from seismic_engine import SparseOps
-    n = (12, 10, 9)
     g = np.random.default_rng(2).standard_normal(int(np.prod(n)))
     got = SparseOps.smooth_gradient(g, *n, sigma)
     ref = osp.smooth_gradient(g, *n, sigma)
