@@ -369,11 +369,14 @@ fd_step_kernel(const FdStepArgs a) {
                                                 reinterpret_cast<typename Vec<VW>::N*>(P + pb + roff[r]));
                 else *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
                 if constexpr (MODE == FD_MODE_SNAPSHOT || MODE == FD_MODE_SNAPSHOT_LAP) {
+                    // read back only by the reverse pass of the segment: keep
+                    // it out of the caches the stencil's planes live in
                     float* snap = a.snap_out + (int64_t)shot * a.ncell;
+                    auto* dst = reinterpret_cast<typename Vec<VW>::N*>(snap + pb + roff[r]);
                     if constexpr (MODE == FD_MODE_SNAPSHOT_LAP)
-                        *reinterpret_cast<vec*>(snap + pb + roff[r]) = lapv[r];
+                        __builtin_nontemporal_store(V::to_native(lapv[r]), dst);
                     else
-                        *reinterpret_cast<vec*>(snap + pb + roff[r]) = q[R][r];
+                        __builtin_nontemporal_store(V::to_native(q[R][r]), dst);
                 }
                 if constexpr (MODE == FD_MODE_ACCUMULATE) {
                     const float* S = a.acc_src + (int64_t)shot * a.ncell;
