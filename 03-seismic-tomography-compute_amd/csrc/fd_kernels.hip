@@ -379,8 +379,10 @@ fd_step_kernel(const FdStepArgs a) {
                         __builtin_nontemporal_store(V::to_native(q[R][r]), dst);
                 }
                 if constexpr (MODE == FD_MODE_ACCUMULATE) {
+                    // the snapshot is read once: non-temporal, like its store
                     const float* S = a.acc_src + (int64_t)shot * a.ncell;
-                    const vec sv = ldv(S, pb + roff[r]);
+                    const vec sv = V::from_native(__builtin_nontemporal_load(
+                        reinterpret_cast<const typename Vec<VW>::N*>(S + pb + roff[r])));
                     const int y = yb + ty * TYR + r;
                     const int64_t ii = (int64_t)shot * a.nint +
                                        ((int64_t)(x - hw) * a.ny + (y - hw)) * a.nz + (z - hw);
