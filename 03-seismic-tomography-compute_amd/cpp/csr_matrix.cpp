@@ -30,6 +30,7 @@ CSRMatrix& CSRMatrix::operator=(CSRMatrix&& o) noexcept {
         values_ = std::move(o.values_);
         cache_blocks_ = std::move(o.cache_blocks_);
         block_size_ = o.block_size_;
+        for (int k = 0; k < 3; ++k) grid_[k] = o.grid_[k];
         dev_ = o.dev_;
         o.dev_ = nullptr;
         o.rows_ = o.cols_ = o.nnz_ = 0;
@@ -49,6 +50,7 @@ CSRMatrix CSRMatrix::clone() const {
     c.values_ = values_.clone();
     c.cache_blocks_ = cache_blocks_;
     c.block_size_ = block_size_;
+    for (int k = 0; k < 3; ++k) c.grid_[k] = grid_[k];
     return c;
 }
 
@@ -62,6 +64,7 @@ void CSRMatrix::assign(int64_t rows, int64_t cols, AlignedVector<int64_t>&& rp, 
     if (rows < 0 || cols < 0 || rp.size() != static_cast<size_t>(rows + 1) || ci.size() != v.size())
         throw std::invalid_argument("CSRMatrix::assign: inconsistent arrays");
     drop_device();
+    if (cols != cols_) grid_[0] = grid_[1] = grid_[2] = 0;   // a grid of the old column count
     rows_ = rows;
     cols_ = cols;
     nnz_ = static_cast<int64_t>(ci.size());
@@ -145,7 +148,18 @@ void* CSRMatrix::device_handle() const {
                "CSRMatrix device copy");
     dev::check(stp_stream_sync(nullptr), "CSRMatrix device copy");
     dev_ = h;
+    if (grid_[0] > 0) dev::check(stp_csr_set_grid(h, grid_[0], grid_[1], grid_[2]), "CSRMatrix::set_cell_grid");
     return dev_;
+}
+
+void CSRMatrix::set_cell_grid(int64_t nx, int64_t ny, int64_t nz) {
+    const bool clear = nx == 0 && ny == 0 && nz == 0;
+    if (!clear && (nx < 1 || ny < 1 || nz < 1 || nx * ny * nz != cols_))
+        throw std::invalid_argument("CSRMatrix::set_cell_grid: nx * ny * nz must equal the column count");
+    grid_[0] = nx;
+    grid_[1] = ny;
+    grid_[2] = nz;
+    if (dev_) dev::check(stp_csr_set_grid(static_cast<stp_csr*>(dev_), nx, ny, nz), "CSRMatrix::set_cell_grid");
 }
 
 static AlignedVector<double> spmv_dev(const CSRMatrix& A, const AlignedVector<double>& x, int transpose) {

@@ -42,6 +42,46 @@ def pmc_traffic(n: int) -> tuple[float | None, str | None]:
     return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
+def omp_threads(k: int) -> None:
+    """Thread count of the oracle's OpenMP loops from here on (the runtime
+    reads OMP_NUM_THREADS only once, when it starts)."""
+    import ctypes
+    from oracle import fd as ofd
+    ofd.lib()                                        # loads libgomp with the oracle
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(k))
+
+
+def host_cpu() -> dict:
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0))}
+
+
+def stream_copy_rate(dev, nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Device-to-device copy of nbytes (torch copy_ on the current stream),
+    bytes read + written per second: the measured STREAM-copy rate."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+
 def cpu_baseline(threads: int, vel_h, n: int, steps: int, sponge: str):
     """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
     on the benchmark workload itself: the same n^3 model, order 8, sponge,
@@ -49,6 +89,7 @@ def cpu_baseline(threads: int, vel_h, n: int, steps: int, sponge: str):
     Returns the rate, the time and the final field and traces (the GPU's
     first `steps` steps are compared with them)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    omp_threads(threads)
     from oracle import fd as ofd
     from seismic_engine import synthetic
     c = ofd.cfg((n, n, n), hw=4, order=8, sponge_width=20, sponge_damping=0.015, sponge=sponge)
@@ -580,6 +621,9 @@ def main() -> int:
                      "kernel": "fd_step_kernel<4,4,128,16,1,256>", "bytes_per_cell_update": BYTES_PER_CELL,
                      "avg_launch_us": round(launch_s * 1e6, 2), "traffic_source": traffic_src},
     }
+    copy_gbs = stream_copy_rate(dev)
+    result["roofline"]["stream_copy_gbs"] = round(copy_gbs, 1)
+    result["roofline"]["frac_of_stream_copy"] = round(achieved / copy_gbs, 4)
 
     if not args.no_rtm:
         log(f"RTM imaging: {n}^3, {nt} steps, recorded traces back-propagated")
@@ -612,6 +656,10 @@ def main() -> int:
         steps = args.cpu_steps
         log(f"cpu baseline: FP64 oracle on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
         gcs, sec, cur_o, tr_o = cpu_baseline(args.cpu_threads, vel_h, n, steps, args.sponge)
+        steps1 = 2
+        log(f"cpu baseline: the same on 1 core, first {steps1} steps")
+        gcs1, sec1, _, _ = cpu_baseline(1, vel_h, n, steps1, args.sponge)
+        omp_threads(args.cpu_threads)
         del vel_h
         tr_g, cur_g = spot
         result["spot_check_512"] = {
@@ -620,9 +668,13 @@ def main() -> int:
         del cur_o, tr_o, cur_g, tr_g
         result["cpu_baseline"] = {
             "value": round(gcs, 5), "unit": "Gcell-updates/s", "cores": args.cpu_threads, "kind": "port",
-            "sample": f"FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x) on the same "
-                      f"{n}^3 smooth random model, order 8, sponge 20, {len(rcv)} receivers: first {steps} of "
-                      f"{nt} steps, {sec:.1f} s"}
+            "sample": f"FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x; gcc -O3 "
+                      f"-march=x86-64-v3 without -ffast-math, since the same build is the parity checker) on the "
+                      f"same {n}^3 smooth random model, order 8, sponge 20, {len(rcv)} receivers: first {steps} of "
+                      f"{nt} steps, {sec:.1f} s",
+            "single_core_value": round(gcs1, 5),
+            "single_core_sample": f"the same on 1 thread, first {steps1} steps, {sec1:.1f} s",
+            "host": host_cpu()}
         log("accuracy: 128^3, 500 steps, FP32 GPU vs FP64 oracle")
         result["accuracy"] = [
             accuracy_check(dev, args.cpu_threads, synthetic.two_layer_velocity(128, 128, 128, split_iz=64),

@@ -72,6 +72,11 @@ public:
     // device copy of this matrix (an stp_csr*, built on first use)
     void* device_handle() const;
 
+    // New: the columns are the cells of an nx x ny x nz grid (x-major, z
+    // fastest); SparseOps::cgls then solves in the Morton solver layout
+    // (stp_csr_set_grid).  0, 0, 0 clears it.
+    void set_cell_grid(int64_t nx, int64_t ny, int64_t nz);
+
 private:
     int64_t rows_ = 0;
     int64_t cols_ = 0;
@@ -82,6 +87,7 @@ private:
     std::vector<RowBlock> cache_blocks_;
     int64_t block_size_ = 8;
     mutable void* dev_ = nullptr;
+    int64_t grid_[3] = {0, 0, 0};
     void drop_device() const;
 };
 

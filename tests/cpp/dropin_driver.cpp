@@ -333,6 +333,24 @@ int cmd_mmap(const std::string& dir) {
     return 0;
 }
 
+// CSRMatrix::set_cell_grid argument checks, host only (no device copy is made)
+int cmd_csrgrid() {
+    CSRMatrix A;
+    AlignedVector<int64_t> rp(3), ci(2);
+    AlignedVector<double> v(2);
+    rp[0] = 0; rp[1] = 1; rp[2] = 2; ci[0] = 0; ci[1] = 23; v[0] = 1.0; v[1] = 2.0;
+    A.assign(2, 24, std::move(rp), std::move(ci), std::move(v));
+    int bad = 0;
+    try { A.set_cell_grid(2, 3, 5); } catch (const std::invalid_argument&) { ++bad; }
+    try { A.set_cell_grid(0, 4, 6); } catch (const std::invalid_argument&) { ++bad; }
+    A.set_cell_grid(2, 3, 4);
+    A.set_cell_grid(0, 0, 0);
+    A.set_cell_grid(4, 3, 2);
+    CSRMatrix B = A.clone();
+    std::printf("rejected=%d accepted=1 clone_ok=%d\n", bad, B.cols() == 24 ? 1 : 0);
+    return 0;
+}
+
 int cmd_health(const std::string& dir) {
     std::vector<double> d = read<double>(dir + "/field.f64");
     const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
@@ -403,6 +421,7 @@ int main(int argc, char** argv) {
         if (cmd == "health") return cmd_health(arg);
         if (cmd == "terrain") return cmd_terrain(arg);
         if (cmd == "mmap") return cmd_mmap(arg);
+        if (cmd == "csrgrid") return cmd_csrgrid();
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

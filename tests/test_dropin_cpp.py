@@ -146,3 +146,9 @@ def test_grid_and_single_process_context():
     assert "index_with_halo(2,3,4) 229" in text       # (2*11 + 3)*9 + 4
     assert "is_halo(0,0,0) 1 is_halo(2,2,2) 0 x_coord(1) 51" in text
     assert "rank 0 size 1 neighbor0 -1 allreduce 2.5" in text
+
+
+def test_csr_set_cell_grid_checks():
+    out = subprocess.run([str(DRIVER), "csrgrid"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert "rejected=2 accepted=1 clone_ok=1" in out.stdout
