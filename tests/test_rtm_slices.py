@@ -51,7 +51,7 @@ def test_frame_roundtrip_layout_and_corruption():
     assert decode_slice_frame(buf[:100]) is None
 
 
-def _ref_slices(fwd, adj, ic, specs):
+def _ref_slices(fwd, adj, ic, specs, HW=HW):
     lib = C.CDLL(str(REFLIB))
     lib.ref_rtm_create.restype = C.c_void_p
     h = C.c_void_p(lib.ref_rtm_create(C.c_int64(N[0]), C.c_int64(N[1]), C.c_int64(N[2]), C.c_int(HW)))
@@ -77,19 +77,20 @@ def _ref_slices(fwd, adj, ic, specs):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not REFLIB.exists(), reason="reference build (oracle/_ref) not present")
-def test_gpu_slices_match_reference(cuda):
+@pytest.mark.parametrize("HW", [0, 3, 4])
+def test_gpu_slices_match_reference(HW, cuda):
     from seismic_engine.rtm import RTMEngine
     rng = np.random.default_rng(1)
     nh = (N[0] + 2 * HW) * (N[1] + 2 * HW) * (N[2] + 2 * HW)
     fwd = rng.standard_normal(nh)
     adj = rng.standard_normal(nh)
     ic = rng.standard_normal(N[0] * N[1] * N[2])
-    fwd[[5, 400, 1000]] = [np.nan, np.inf, -np.inf]
+    fwd[[5, nh // 2, nh - 7]] = [np.nan, np.inf, -np.inf]
     adj[[7, 401]] = np.nan
     ic[[0, 3]] = np.nan
     specs = [SliceSpec(SliceAxis.X, 2), SliceSpec(SliceAxis.Y, -1), SliceSpec(SliceAxis.Z, 10),
              SliceSpec(SliceAxis.VOLUME, -1), SliceSpec(SliceAxis.Z, -1)]
-    ref = _ref_slices(fwd, adj, ic, specs)
+    ref = _ref_slices(fwd, adj, ic, specs, HW)
     e = RTMEngine(*N, halo_width=HW)
     e.set_slice_specs(specs)
     got = []
