@@ -208,6 +208,26 @@ int build_solver_layout(stp_csr* h, hipStream_t st) {
     return STP_OK;
 }
 
+template <typename T>
+__global__ void permute_kernel(int n, const T* __restrict__ src, const int* __restrict__ idx, T* __restrict__ dst,
+                               int scatter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (scatter)
+        dst[idx[i]] = src[i];
+    else
+        dst[i] = src[idx[i]];
+}
+
+template <typename T>
+hipError_t permute(int n, const T* src, const int* idx, T* dst, bool scatter, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(permute_kernel<T>, dim3(blocks(n)), dim3(256), 0, st, n, src, idx, dst, scatter ? 1 : 0);
+    return hipGetLastError();
+}
+template hipError_t permute<float>(int, const float*, const int*, float*, bool, hipStream_t);
+template hipError_t permute<double>(int, const double*, const int*, double*, bool, hipStream_t);
+
 hipError_t gather_f64(int n, const double* src, const int* idx, double* dst, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_f64_kernel, dim3(blocks(n)), dim3(256), 0, st, n, src, idx, dst);

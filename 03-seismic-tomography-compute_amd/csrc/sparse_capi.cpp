@@ -148,8 +148,54 @@ int stp_csr_export(const stp_csr* h, int32_t transpose, int32_t* rp, int32_t* ci
     return STP_OK;
 }
 
+}  // extern "C"
+
+// y = A x / A^T x through the solver layout (when it is built): x is permuted
+// into layout order, multiplied by the renumbered copy and the result
+// permuted back.  Temporaries come from the stream-ordered pool, so products
+// on one handle may run on several streams at once.
+template <typename T>
+static int layout_spmv(const stp_csr* h, int transpose, const T* x, T* y, hipStream_t st) {
+    const int rows = h->rows, cols = h->cols;
+    const int nin = transpose ? rows : cols, nout = transpose ? cols : rows;
+    T *xl = nullptr, *yl = nullptr;
+    STP_HIP(hipMallocAsync(reinterpret_cast<void**>(&xl), sizeof(T) * (size_t)(nin > 0 ? nin : 1), st));
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&yl), sizeof(T) * (size_t)(nout > 0 ? nout : 1), st);
+    if (e == hipSuccess) {
+        if (!transpose) {   // x'[cperm[j]] = x[j]; y[rsrc[k]] = (A' x')[k]
+            e = stp::permute<T>(cols, x, h->cperm, xl, true, st);
+            if (e == hipSuccess) {
+                if constexpr (sizeof(T) == 4)
+                    e = stp::spmv_f32(h->lW, rows, h->lrp, h->lci, h->lval, xl, yl, nullptr, nullptr, st);
+                else
+                    e = stp::spmv_f64(h->lW, rows, h->lrp, h->lci, h->lval, xl, yl, 1.0, nullptr, 0.0, nullptr,
+                                      nullptr, st);
+            }
+            if (e == hipSuccess) e = stp::permute<T>(rows, yl, h->rsrc, y, true, st);
+        } else {            // x'[k] = x[rsrc[k]]; y[j] = (A'^T x')[cperm[j]]
+            e = stp::permute<T>(rows, x, h->rsrc, xl, false, st);
+            if (e == hipSuccess) {
+                if constexpr (sizeof(T) == 4)
+                    e = stp::spmv_f32(h->lWT, cols, h->ltrp, h->ltci, h->ltval, xl, yl, nullptr, nullptr, st);
+                else
+                    e = stp::spmv_f64(h->lWT, cols, h->ltrp, h->ltci, h->ltval, xl, yl, 1.0, nullptr, 0.0, nullptr,
+                                      nullptr, st);
+            }
+            if (e == hipSuccess) e = stp::permute<T>(cols, yl, h->cperm, y, false, st);
+        }
+    }
+    const hipError_t e1 = hipFreeAsync(xl, st);
+    const hipError_t e2 = yl ? hipFreeAsync(yl, st) : hipSuccess;
+    if (e == hipSuccess) e = e1 != hipSuccess ? e1 : e2;
+    if (e != hipSuccess) return stp::hip_fail(e, "sparse product in the solver layout");
+    return STP_OK;
+}
+
+extern "C" {
+
 int stp_csr_spmv(const stp_csr* h, int32_t transpose, const float* x, float* y, void* stream) {
     if (!h || !x || !y) return fail(STP_ERR_INVALID, "stp_csr_spmv: null argument");
+    if (h->lay_ready) return layout_spmv<float>(h, transpose, x, y, S(stream));
     if (transpose)
         STP_HIP(stp::spmv_f32(h->WT, h->cols, h->trp, h->tci, h->tval, x, y, nullptr, nullptr, S(stream)));
     else
@@ -159,6 +205,7 @@ int stp_csr_spmv(const stp_csr* h, int32_t transpose, const float* x, float* y, 
 
 int stp_csr_spmv_f64(const stp_csr* h, int32_t transpose, const double* x, double* y, void* stream) {
     if (!h || !x || !y) return fail(STP_ERR_INVALID, "stp_csr_spmv_f64: null argument");
+    if (h->lay_ready) return layout_spmv<double>(h, transpose, x, y, S(stream));
     if (transpose)
         STP_HIP(stp::spmv_f64(h->WT, h->cols, h->trp, h->tci, h->tval, x, y, 1.0, nullptr, 0.0, nullptr, nullptr,
                               S(stream)));

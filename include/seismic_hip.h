@@ -169,7 +169,10 @@ int stp_csr_destroy(stp_csr* A);
 int stp_csr_shape(const stp_csr* A, int64_t* rows, int64_t* cols, int64_t* nnz);
 int stp_csr_export(const stp_csr* A, int32_t transpose, int32_t* row_ptr_dev, int32_t* col_dev, float* val_dev,
                    void* stream);
-/* y = A x (transpose = 0) or A^T x; FP32 accumulation */
+/* y = A x (transpose = 0) or A^T x; FP32 accumulation.  Once the solver
+ * layout of A is built (stp_csr_prepare_layout, or a CGLS solve of a matrix
+ * with a grid) both products run on it, with x and y permuted on the way in
+ * and out: the same sums up to the order of FP32 additions. */
 int stp_csr_spmv(const stp_csr* A, int32_t transpose, const float* x_dev, float* y_dev, void* stream);
 /* same with FP64 x, FP64 accumulation and FP64 y (values stay FP32) */
 int stp_csr_spmv_f64(const stp_csr* A, int32_t transpose, const double* x_dev, double* y_dev, void* stream);

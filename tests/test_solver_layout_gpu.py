@@ -92,3 +92,19 @@ def test_layout_with_empty_rows_and_grid_checks(cuda):
     A.set_grid((4, 5, 6))
     x4, i4, _ = A.cgls(torch.as_tensor(b, device="cuda"), LAM, 1e-8, 5000, "jacobi")
     assert i4.converged and rel(x4.cpu(), xd) < 1e-5
+
+
+@pytest.mark.parametrize("n", [(24, 24, 24), (20, 12, 7)])
+def test_products_through_the_layout_match_the_given_order(n, cuda):
+    pts = synthetic.ray_endpoints(5 * int(np.prod(n)) // 4, n, seed=45)
+    A = DeviceCSR.from_rays(pts, n)
+    rng = np.random.default_rng(46)
+    x = torch.as_tensor(rng.standard_normal(A.cols), device="cuda")
+    u = torch.as_tensor(rng.standard_normal(A.rows), device="cuda")
+    plain = [A.matvec(x), A.matvec(u, True), A.matvec(x.float()), A.matvec(u.float(), True)]
+    A.prepare_layout()
+    lay = [A.matvec(x), A.matvec(u, True), A.matvec(x.float()), A.matvec(u.float(), True)]
+    for k, (p, q) in enumerate(zip(plain, lay)):
+        p, q = p.double().cpu().numpy(), q.double().cpu().numpy()
+        tol = 1e-13 if k < 2 else 1e-6          # FP64 / FP32 accumulation, different summation order
+        assert rel(q, p) < tol, (k, rel(q, p))
