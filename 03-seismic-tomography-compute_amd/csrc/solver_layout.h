@@ -7,7 +7,6 @@
 namespace stp {
 int build_solver_layout(stp_csr* h, hipStream_t st);   // no-op without a grid or when built
 void free_solver_layout(stp_csr* h);
-hipError_t gather_f64(int n, const double* src, const int* idx, double* dst, hipStream_t st);   // dst[i] = src[idx[i]]
 // dst[idx[i]] = src[i] (scatter) or dst[i] = src[idx[i]] (gather), i < n
 template <typename T>
 hipError_t permute(int n, const T* src, const int* idx, T* dst, bool scatter, hipStream_t st);

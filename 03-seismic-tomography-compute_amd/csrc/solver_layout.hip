@@ -107,12 +107,6 @@ __global__ void fill_kernel(int rows, const int* __restrict__ rsrc, const int* _
     }
 }
 
-__global__ void gather_f64_kernel(int n, const double* __restrict__ src, const int* __restrict__ idx,
-                                  double* __restrict__ dst) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[idx[i]];
-}
-
 template <typename K>
 hipError_t sort_pairs(K* k0, K* k1, int* v0, int* v1, int n, int end_bit, hipStream_t st) {
     size_t tb = 0;
@@ -227,12 +221,6 @@ hipError_t permute(int n, const T* src, const int* idx, T* dst, bool scatter, hi
 }
 template hipError_t permute<float>(int, const float*, const int*, float*, bool, hipStream_t);
 template hipError_t permute<double>(int, const double*, const int*, double*, bool, hipStream_t);
-
-hipError_t gather_f64(int n, const double* src, const int* idx, double* dst, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_f64_kernel, dim3(blocks(n)), dim3(256), 0, st, n, src, idx, dst);
-    return hipGetLastError();
-}
 
 }  // namespace stp
 

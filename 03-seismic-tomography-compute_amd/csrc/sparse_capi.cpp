@@ -318,7 +318,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     const int W = lay ? h->lW : h->W, WT = lay ? h->lWT : h->WT;
     double* const x_out = x;
     if (lay) {
-        STP_HIP(stp::gather_f64(rows, b, h->rsrc, h->lb, st));
+        STP_HIP(stp::permute<double>(rows, b, h->rsrc, h->lb, false, st));
         b = h->lb;
         x = h->lx;
     }
@@ -415,7 +415,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         if (rc != STP_OK) return rc;
         STP_HIP(hipMemsetAsync(&h->sc->done, 0, sizeof(int), st));
     }
-    if (lay) STP_HIP(stp::gather_f64(cols, x, h->cperm, x_out, st));
+    if (lay) STP_HIP(stp::permute<double>(cols, x, h->cperm, x_out, false, st));
     if (info) {
         info->iterations = it;
         info->converged = (hs.done == 1 || hs.s0 == 0) && true_rel < tol;
