@@ -191,7 +191,8 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
     A.prepare_layout()
     torch.cuda.synchronize()
     out["layout_seconds"] = round(time.perf_counter() - t0, 5)
-    # SpMV alone: 8 B/nnz + row offsets + y + gathered x (counted once)
+    # SpMV alone (through the solver layout built above, the two vector
+    # permutations included): 8 B/nnz + row offsets + y + gathered x (counted once)
     x32 = s_true.float()
     y32 = torch.empty(A.rows, dtype=torch.float32, device=dev)
     for _ in range(3):
