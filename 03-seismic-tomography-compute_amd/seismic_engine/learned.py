@@ -2,7 +2,7 @@
 libseismic_hip.so).
 
 The reference preconditions CG with Jacobi only
-(src/cpp/sparse/sparse_ops.cpp:52-113); that stays the fallback and the judge.
+(src/cpp/sparse/sparse_ops.cpp:52-113); that stays the fallback and the yardstick.
 Here a 3-level 3-D convolutional network (16/32/64 channels, 3x3x3, ReLU,
 BF16 on MFMA with FP32 accumulation) maps the Jacobi-scaled local couplings of
 N = A^T A + lam I to a per-cell scale and per-edge weights, and CGLS runs with

@@ -68,7 +68,7 @@ def test_solver_forward_step_adjoint_match_oracle(tmp_path, cuda):
     assert rel(f64(tmp_path, "cur_adj.f64"), cur) < 1e-5
 
 
-@pytest.mark.skip(reason="the driver's sparse command ended with SIGSEGV on the GPU box; its host code is "
+@pytest.mark.skip(reason="the driver's sparse command ended with SIGSEGV in an MI355X run; its host code is "
                          "clean under AddressSanitizer on a CPU stand-in of the C ABI and the cause is not yet "
                          "shown, so the case stays off the GPU (the same C ABI calls pass through Python in "
                          "test_sparse_gpu.py)")
