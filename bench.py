@@ -346,6 +346,40 @@ def learned_heldout(dev, n: int = 96, lam: float = 1e-2, tol: float = 1e-6, net=
     return out
 
 
+def network_mfma(dev, n: int = 64, reps: int = 20):
+    """MFMA utilisation of the learned preconditioner's network at an n^3
+    system: each layer's 3x3x3 BF16 convolution (FP32 accumulation) timed
+    alone with HIP events on the launching stream; 2 cin cout 27 FLOP per cell
+    of the layer's level (full, half, quarter, half, full, full grid) against
+    the 2.5 PFLOP/s dense BF16 peak."""
+    import torch
+    from seismic_engine.learned import LAYERS, conv3d_bf16
+    levels = (0, 1, 2, 1, 0, 0)
+    gen = torch.Generator(device=dev).manual_seed(11)
+    layers, tot_f, tot_t = [], 0.0, 0.0
+    for (cin, cout), lv in zip(LAYERS, levels):
+        m = n >> lv
+        x = torch.randn(m ** 3, cin, device=dev, generator=gen).to(torch.bfloat16).view(torch.int16)
+        w = 0.1 * torch.randn(cout, 27, cin, device=dev, generator=gen)
+        conv3d_bf16(x, w, (m, m, m))
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            conv3d_bf16(x, w, (m, m, m))
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3 / reps
+        f = 2.0 * cin * cout * 27 * m ** 3
+        tot_f += f
+        tot_t += t
+        layers.append({"cin": cin, "cout": cout, "grid": m, "us": round(t * 1e6, 1),
+                       "tflop_s": round(f / t / 1e12, 2)})
+    return {"grid": n, "dtype": "bf16 in, fp32 accumulate", "layers": layers,
+            "forward_us": round(tot_t * 1e6, 1), "tflop_s": round(tot_f / tot_t / 1e12, 2),
+            "frac_bf16_dense_peak": round(tot_f / tot_t / 1e12 / 2500.0, 5)}
+
+
 def aux_workload(dev, n: int = 512, cpu_n: int = 128, hw: int = 4, reps: int = 10):
     """Per-call rate of the inversion loop's gradient post-processing, the
     terrain boundary and RTM slice extraction at n^3, FP64: HIP events on the
@@ -555,8 +589,10 @@ def main() -> int:
         print(json.dumps({"aux": aux_workload(dev)}), flush=True)
         return 0
     if args.cg_only:
-        print(json.dumps({"cg": cg_workload(dev, kinds=tuple(args.cg_kinds.split(",")),
-                                                     given_order=not args.no_cg_given_order)}), flush=True)
+        cg = cg_workload(dev, kinds=tuple(args.cg_kinds.split(",")), given_order=not args.no_cg_given_order)
+        if "learned" in args.cg_kinds:
+            cg["network_mfma"] = network_mfma(dev)
+        print(json.dumps({"cg": cg}), flush=True)
         return 0
     n, nt, hw = args.n, args.nt, 4
     log(f"building {n}^3 velocity (seed 2026, sigma 20) on the GPU")
@@ -660,6 +696,8 @@ def main() -> int:
     if not args.no_cg:
         log("tomography solve: 200k rays, 64^3, CGLS to 1e-6")
         result["cg"] = cg_workload(dev)
+        log("learned preconditioner network: per-layer BF16 MFMA convolutions at 64^3")
+        result["cg"]["network_mfma"] = network_mfma(dev)
 
     if not args.no_cpu_baseline:
         if args.no_grad:
