@@ -146,6 +146,12 @@ void* CSRMatrix::device_handle() const {
     stp_csr* h = nullptr;
     dev::check(stp_csr_create(rows_, cols_, nnz_, rp32.get(), ci32.get(), v32.get(), &h, nullptr),
                "CSRMatrix device copy");
+    // the FP64 values serve SparseOps' CG, which the reference runs in FP64
+    const int rc = stp_csr_set_values_f64(h, v.get(), nullptr);
+    if (rc != STP_OK) {
+        stp_csr_destroy(h);
+        dev::check(rc, "CSRMatrix device copy");
+    }
     dev::check(stp_stream_sync(nullptr), "CSRMatrix device copy");
     dev_ = h;
     if (grid_[0] > 0) dev::check(stp_csr_set_grid(h, grid_[0], grid_[1], grid_[2]), "CSRMatrix::set_cell_grid");

@@ -11,6 +11,7 @@ struct stp_csr {
     int64_t nnz = 0;
     int *rp = nullptr, *ci = nullptr, *trp = nullptr, *tci = nullptr;
     float *val = nullptr, *tval = nullptr;
+    double* val64 = nullptr;   // optional FP64 values of A (square CG in FP64)
     int W = 64, WT = 64;   // lanes per row for A and A^T
     // workspace
     double *r = nullptr, *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;

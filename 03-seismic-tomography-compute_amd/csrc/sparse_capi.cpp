@@ -40,7 +40,7 @@ hipError_t grow(T** p, size_t* cap, size_t n) {
 
 
 static void csr_free(stp_csr* h) {
-    void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
+    void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->val64, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
                   h->p, h->q, h->s, h->z, h->sp, h->dinv, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
     for (void* p : ps)
         if (p) hipFree(p);
@@ -216,6 +216,19 @@ int stp_csr_spmv_f64(const stp_csr* h, int32_t transpose, const double* x, doubl
 }
 
 // ------------------------------------------------------------------ square CG / Jacobi-PCG
+int stp_csr_set_values_f64(stp_csr* h, const double* val64, void* stream) {
+    if (!h) return fail(STP_ERR_INVALID, "stp_csr_set_values_f64: null handle");
+    if (!val64) {
+        if (h->val64) hipFree(h->val64);
+        h->val64 = nullptr;
+        return STP_OK;
+    }
+    if (!h->val64) STP_HIP(hipMalloc(&h->val64, sizeof(double) * (h->nnz ? h->nnz : 1)));
+    if (h->nnz)
+        STP_HIP(hipMemcpyAsync(h->val64, val64, sizeof(double) * h->nnz, hipMemcpyDeviceToDevice, S(stream)));
+    return STP_OK;
+}
+
 int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int32_t jacobi, double* x,
            double* history, stp_solve_info* info, void* stream) {
     stp_csr* h = const_cast<stp_csr*>(hc);
@@ -224,6 +237,7 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
     hipStream_t st = S(stream);
     const int n = h->rows;
     const int nbd = stp::cg_blocks(n);
+    const double* v64 = h->val64;
     STP_HIP(grow(&h->dp, &h->c_dp, n));
     STP_HIP(grow(&h->dap, &h->c_dap, n));
     STP_HIP(grow(&h->dz, &h->c_dz, n));
@@ -234,14 +248,20 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
     STP_HIP(grow(&h->sc, &h->c_sc, 1));
     double* dinv = nullptr;
     if (jacobi) {
-        STP_HIP(stp::launch_diag_inv(n, h->rp, h->ci, h->val, h->ddinv, st));
+        if (v64) STP_HIP(stp::launch_diag_inv_f64v(n, h->rp, h->ci, v64, h->ddinv, st));
+        else STP_HIP(stp::launch_diag_inv(n, h->rp, h->ci, h->val, h->ddinv, st));
         dinv = h->ddinv;
     }
+    // A p with the most precise values the handle holds
+    auto matvec = [&](const double* in, double* out, const int* done) -> hipError_t {
+        if (v64) return stp::spmv_f64v(h->W, n, h->rp, h->ci, v64, in, out, done, st);
+        return stp::spmv_f64(h->W, n, h->rp, h->ci, h->val, in, out, 1.0, nullptr, 0.0, nullptr, done, st);
+    };
     double* z = jacobi ? h->dz : h->dr;
     STP_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, st));
     STP_HIP(hipMemcpyAsync(h->dr, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     if (jacobi) {
-        // z = D^-1 r via the update kernel with alpha = 0 is not available before init; use a copy + scale
+        // z = D^-1 r and the r.r / r.z partials through the update kernel with alpha = 0
         STP_HIP(hipMemsetAsync(h->sc, 0, sizeof(stp::SolverScalars), st));
         STP_HIP(hipMemsetAsync(h->dap, 0, sizeof(double) * n, st));
         STP_HIP(stp::launch_cg_update(h->sc, n, x, h->dr, h->dap, h->dap, dinv, h->dz, h->part0, h->part1, st));
@@ -256,7 +276,7 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
     stp::SolverScalars hs;
     std::memset(&hs, 0, sizeof hs);
     for (; it < max_iter; ++it) {
-        STP_HIP(stp::spmv_f64(h->W, n, h->rp, h->ci, h->val, h->dp, h->dap, 1.0, nullptr, 0.0, nullptr, done, st));
+        STP_HIP(matvec(h->dp, h->dap, done));
         STP_HIP(stp::launch_dot_partial(done, n, h->dp, h->dap, h->part0, st));
         STP_HIP(stp::launch_cg_alpha(h->sc, h->part0, nbd, st));
         STP_HIP(stp::launch_cg_update(h->sc, n, x, h->dr, h->dp, h->dap, dinv, h->dz, h->part0, h->part1, st));
@@ -269,12 +289,20 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
         }
     }
     STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+    // true residual b - A x (into dap: the iteration no longer needs it)
+    if (v64) STP_HIP(stp::residual_f64v(h->W, n, h->rp, h->ci, v64, x, b, h->dap, st));
+    else STP_HIP(stp::residual(h->W, n, h->rp, h->ci, h->val, x, b, h->dap, st));
+    STP_HIP(stp::launch_dot_partial(nullptr, n, h->dap, h->dap, h->part0, st));
+    std::vector<double> pr(nbd);
+    STP_HIP(hipMemcpyAsync(pr.data(), h->part0, sizeof(double) * nbd, hipMemcpyDeviceToHost, st));
     STP_HIP(hipStreamSynchronize(st));
+    double rr = 0.0;
+    for (double v : pr) rr += v;
     if (info) {
         info->iterations = hs.iters;
         info->converged = hs.done == 1;
         info->rel_residual = hs.rel;
-        info->true_rel_residual = hs.rel;
+        info->true_rel_residual = hs.s0 > 1e-30 ? std::sqrt(rr) / hs.s0 : 0.0;
         info->replacements = 0;
     }
     return STP_OK;
@@ -411,9 +439,8 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         true_rel = hs.s0 > 0 ? std::sqrt(ss) / hs.s0 : 0.0;
         if (hs.done != 1 || true_rel < tol || it >= max_iter || replacements >= 8) break;
         ++replacements;
-        rc = start(0);
+        rc = start(0);   // cgls_init clears the done flag before the direction kernel runs
         if (rc != STP_OK) return rc;
-        STP_HIP(hipMemsetAsync(&h->sc->done, 0, sizeof(int), st));
     }
     if (lay) STP_HIP(stp::permute<double>(cols, x, h->cperm, x_out, false, st));
     if (info) {

@@ -23,6 +23,11 @@ hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const 
                            double* part_ss, const int* done, hipStream_t st);
 hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
                     double* r, hipStream_t st);
+// FP64-valued variants (the square CG drop-in when FP64 values are attached)
+hipError_t spmv_f64v(int W, int rows, const int* rp, const int* ci, const double* val, const double* x, double* y,
+                     const int* done, hipStream_t st);
+hipError_t residual_f64v(int W, int rows, const int* rp, const int* ci, const double* val, const double* x,
+                         const double* b, double* r, hipStream_t st);
 
 hipError_t rays_count(int64_t nrays, const double* pts, const int64_t* n, double h, int* counts, hipStream_t st);
 hipError_t rays_fill(int64_t nrays, const double* pts, const int64_t* n, double h, const int* rp, int* cols,
@@ -46,6 +51,8 @@ hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const
 hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st);
 hipError_t launch_normal_diag(int n, const int* rp, const float* val, double lam, float* dinv, hipStream_t st);
 hipError_t launch_diag_inv(int n, const int* rp, const int* ci, const float* val, double* dinv, hipStream_t st);
+hipError_t launch_diag_inv_f64v(int n, const int* rp, const int* ci, const double* val, double* dinv,
+                                hipStream_t st);
 int cg_blocks(int n);
 hipError_t launch_dot_partial(const int* done, int n, const double* a, const double* b, double* part, hipStream_t st);
 hipError_t launch_cg_alpha(SolverScalars* sc, const double* part, int n, hipStream_t st);

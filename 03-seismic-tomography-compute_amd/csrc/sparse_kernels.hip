@@ -34,14 +34,14 @@ constexpr int MAX_BLOCKS = 2048;
 // instead of a chain of dependent load pairs.  Slots past the row end load
 // the row's first entry and contribute an exact zero, which leaves the sum
 // (and its order) unchanged.
-template <int W, typename AT, typename XT>
+template <int W, typename AT, typename XT, typename VT = float>
 __device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const int* __restrict__ ci,
-                                      const float* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
+                                      const VT* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
     constexpr int U = (128 / W) < 1 ? 1 : (128 / W);
     AT acc = 0;
     const int b = rp[row], e = rp[row + 1];
     for (int k0 = b + lane; k0 < e; k0 += U * W) {
-        float v[U];
+        VT v[U];
         int c[U];
         bool ok[U];
 #pragma unroll
@@ -75,9 +75,9 @@ __device__ __forceinline__ void block_partial(double v, double* part) {
     if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART>
+template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART, typename VT = float>
 __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                    const float* __restrict__ val, const XT* __restrict__ x,
+                                                    const VT* __restrict__ val, const XT* __restrict__ x,
                                                     YT* __restrict__ y, double alpha, const double* __restrict__ wv,
                                                     double lam, double* __restrict__ part, const int* done) {
     if (done && *done) return;
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restri
     for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
         const int row = row0 + sub;
         if (row < rows) {
-            const AT acc = row_dot<W, AT, XT>(rp, ci, val, x, row, lane);
+            const AT acc = row_dot<W, AT, XT, VT>(rp, ci, val, x, row, lane);
             if (lane == 0) {
                 double v = alpha * (double)acc;
                 if (FUSE_W) v -= lam * wv[row];
@@ -100,13 +100,13 @@ __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restri
     if (PART) block_partial(ps, part);
 }
 
-template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART>
-static hipError_t spmv_w(int rows, const int* rp, const int* ci, const float* val, const XT* x, YT* y, double alpha,
+template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART, typename VT = float>
+static hipError_t spmv_w(int rows, const int* rp, const int* ci, const VT* val, const XT* x, YT* y, double alpha,
                          const double* wv, double lam, double* part, const int* done, hipStream_t st) {
     const int nb = spmv_blocks(rows, W);
     if (rows == 0) return hipSuccess;
-    hipLaunchKernelGGL((spmv_kernel<W, XT, AT, YT, FUSE_W, PART>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, y,
-                       alpha, wv, lam, part, done);
+    hipLaunchKernelGGL((spmv_kernel<W, XT, AT, YT, FUSE_W, PART, VT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val,
+                       x, y, alpha, wv, lam, part, done);
     return hipGetLastError();
 }
 
@@ -168,6 +168,25 @@ hipError_t spmv_f64(int W, int rows, const int* rp, const int* ci, const float* 
     }
     if (part) { STP_SPMV_DISPATCH(double, double, double, false, true) }
     STP_SPMV_DISPATCH(double, double, double, false, false)
+}
+
+// FP64 values, FP64 x and accumulation: the square CG of the reference's
+// all-FP64 arithmetic (12 bytes per nonzero instead of 8).
+hipError_t spmv_f64v(int W, int rows, const int* rp, const int* ci, const double* val, const double* x, double* y,
+                     const int* done, hipStream_t st) {
+    const double alpha = 1.0, lam = 0.0;
+    const double* wv = nullptr;
+    double* part = nullptr;
+    switch (W) {
+        case 64: return spmv_w<64, double, double, double, false, false, double>(rows, rp, ci, val, x, y, alpha, wv,
+                                                                                  lam, part, done, st);
+        case 32: return spmv_w<32, double, double, double, false, false, double>(rows, rp, ci, val, x, y, alpha, wv,
+                                                                                  lam, part, done, st);
+        case 16: return spmv_w<16, double, double, double, false, false, double>(rows, rp, ci, val, x, y, alpha, wv,
+                                                                                  lam, part, done, st);
+        default: return spmv_w<8, double, double, double, false, false, double>(rows, rp, ci, val, x, y, alpha, wv,
+                                                                                 lam, part, done, st);
+    }
 }
 
 // s = A^T r - lam*x with FP64 r and x, FP32 output s, z = dinv*s, partials of s*z and s*s
@@ -324,7 +343,9 @@ __global__ void flex_dots_kernel(const int* done, int n, const float* z, const f
     if (threadIdx.x == 0) { part_zs[blockIdx.x] = red[0][0]; part_fl[blockIdx.x] = red[1][0]; }
 }
 
-// gamma <- sum(psz), s0 <- sqrt(sum(pss)) (initialisation) or only gamma (restart)
+// gamma <- sum(psz), s0 <- sqrt(sum(pss)) (initialisation) or only gamma
+// (restart from a recomputed residual: the done flag of the finished pass is
+// cleared here, so the direction kernel that follows runs)
 __global__ void cgls_init_kernel(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
                                  int set_s0) {
     const double sz = block_sum(psz, nsz);
@@ -332,6 +353,7 @@ __global__ void cgls_init_kernel(SolverScalars* sc, const double* psz, int nsz, 
     if (threadIdx.x == 0) {
         sc->gamma = sz;
         if (set_s0) { sc->s0 = sqrt(ss); sc->done = sc->s0 > 0.0 ? 0 : 1; sc->iters = 0; }
+        else sc->done = 0;
         sc->rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
     }
 }
@@ -342,32 +364,47 @@ __global__ void copy_f32_kernel(int n, const float* a, float* b) {
 }
 
 // r = b - A x  (FP64 x, FP64 accumulation)
-template <int W>
+template <int W, typename VT>
 __global__ void __launch_bounds__(256) residual_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                       const float* __restrict__ val, const double* __restrict__ x,
+                                                       const VT* __restrict__ val, const double* __restrict__ x,
                                                        const double* __restrict__ b, double* __restrict__ r) {
     constexpr int RPB = 256 / W;
     const int lane = threadIdx.x % W, sub = threadIdx.x / W;
     for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
         const int row = row0 + sub;
         if (row < rows) {
-            const double acc = row_dot<W, double, double>(rp, ci, val, x, row, lane);
+            const double acc = row_dot<W, double, double, VT>(rp, ci, val, x, row, lane);
             if (lane == 0) r[row] = b[row] - acc;
         }
     }
 }
 
-hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
-                    double* r, hipStream_t st) {
+template <typename VT>
+static hipError_t residual_t(int W, int rows, const int* rp, const int* ci, const VT* val, const double* x,
+                             const double* b, double* r, hipStream_t st) {
     const int nb = spmv_blocks(rows, W);
     if (rows == 0) return hipSuccess;
     switch (W) {
-        case 64: hipLaunchKernelGGL(residual_kernel<64>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
-        case 32: hipLaunchKernelGGL(residual_kernel<32>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
-        case 16: hipLaunchKernelGGL(residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
-        default: hipLaunchKernelGGL(residual_kernel<8>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r); break;
+        case 64: hipLaunchKernelGGL((residual_kernel<64, VT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r);
+            break;
+        case 32: hipLaunchKernelGGL((residual_kernel<32, VT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r);
+            break;
+        case 16: hipLaunchKernelGGL((residual_kernel<16, VT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r);
+            break;
+        default: hipLaunchKernelGGL((residual_kernel<8, VT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, x, b, r);
+            break;
     }
     return hipGetLastError();
+}
+
+hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
+                    double* r, hipStream_t st) {
+    return residual_t<float>(W, rows, rp, ci, val, x, b, r, st);
+}
+
+hipError_t residual_f64v(int W, int rows, const int* rp, const int* ci, const double* val, const double* x,
+                         const double* b, double* r, hipStream_t st) {
+    return residual_t<double>(W, rows, rp, ci, val, x, b, r, st);
 }
 
 // Jacobi for A^T A + lam I: dinv_j = 1 / (sum_i a_ij^2 + lam), from the rows of A^T
@@ -390,7 +427,8 @@ __global__ void __launch_bounds__(256) normal_diag_kernel(int n, const int* rp, 
 
 // Jacobi for a square A (reference jacobi_preconditioned_cg): first diagonal
 // entry of each row with |a| > 1e-30, else 1
-__global__ void diag_inv_kernel(int n, const int* rp, const int* ci, const float* val, double* dinv) {
+template <typename VT>
+__global__ void diag_inv_kernel(int n, const int* rp, const int* ci, const VT* val, double* dinv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double d = 1.0;
@@ -728,7 +766,13 @@ hipError_t launch_normal_diag(int n, const int* rp, const float* val, double lam
     return hipGetLastError();
 }
 hipError_t launch_diag_inv(int n, const int* rp, const int* ci, const float* val, double* dinv, hipStream_t st) {
-    hipLaunchKernelGGL(diag_inv_kernel, dim3(nb256(n)), dim3(256), 0, st, n, rp, ci, val, dinv);
+    hipLaunchKernelGGL(diag_inv_kernel<float>, dim3(nb256(n)), dim3(256), 0, st, n, rp, ci, val, dinv);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag_inv_f64v(int n, const int* rp, const int* ci, const double* val, double* dinv,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(diag_inv_kernel<double>, dim3(nb256(n)), dim3(256), 0, st, n, rp, ci, val, dinv);
     return hipGetLastError();
 }
 int cg_blocks(int n) {

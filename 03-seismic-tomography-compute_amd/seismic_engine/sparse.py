@@ -208,6 +208,8 @@ class CSRMatrix:
             call("stp_csr64_to_f32", C.c_int64(self._rows), C.c_int64(nnz), ptr(self._rp), ptr(self._ci),
                  ptr(self._val), ptr(rp32), ptr(ci32), ptr(v32), stream_handle())
             self._dev = DeviceCSR.from_device(self._rows, self._cols, rp32, ci32, v32, nnz)
+            # FP64 values for the reference's all-FP64 CG (SparseOps.conjugate_gradient)
+            call("stp_csr_set_values_f64", self._dev._h, ptr(self._val), stream_handle())
         return self._dev
 
     # ---- reference methods

@@ -188,11 +188,20 @@ int stp_csr_set_grid(stp_csr* A, int64_t nx, int64_t ny, int64_t nz);
 /* Build that layout now (no-op without a grid or when built).  Synchronises. */
 int stp_csr_prepare_layout(stp_csr* A, void* stream);
 
+/* Attach the FP64 values of A (device, nnz entries in the order of the
+ * values given to stp_csr_create) so that stp_cg runs in the reference's
+ * all-FP64 arithmetic (CSRMatrix keeps FP64 values: csr_matrix.h:68).  The
+ * FP32 copy still serves SpMV and CGLS.  Copies; val64_dev = NULL detaches. */
+int stp_csr_set_values_f64(stp_csr* A, const double* val64_dev, void* stream);
+
 /* SparseOps::conjugate_gradient / jacobi_preconditioned_cg on a square A:
  * x0 = 0, stop when |r|/|b| < tol (sparse_ops.cpp:39, 97-98), breakdown when
  * |p.Ap| < 1e-30; Jacobi = first diagonal entry with |a| > 1e-30, else 1.
- * FP64 vectors and dots, FP32 matrix values.  history_dev (optional) gets
- * |r|/|b| after each iteration. */
+ * FP64 vectors and dots; FP64 matrix values when attached
+ * (stp_csr_set_values_f64), else the FP32 values.  On exit the residual
+ * b - A x is recomputed with those values: info->true_rel_residual is
+ * |b - A x| / |b|, info->rel_residual the recursive one.  history_dev
+ * (optional) gets the recursive |r|/|b| after each iteration. */
 int stp_cg(const stp_csr* A, const double* b_dev, int32_t max_iter, double tol, int32_t jacobi, double* x_dev,
            double* history_dev, stp_solve_info* info, void* stream);
 
