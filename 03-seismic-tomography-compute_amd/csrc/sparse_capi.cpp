@@ -9,7 +9,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "capi_common.h"
@@ -321,21 +323,38 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
 // A matrix whose columns are grid cells (stp_csr_from_rays, stp_csr_set_grid)
 // is solved in its solver layout: b is gathered into layout row order, the
 // iteration runs on the renumbered copy and x is gathered back to cell order.
-int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t max_iter, int32_t precond,
-             void* precond_handle, double* x, double* history, stp_solve_info* info, void* stream) {
-    stp_csr* h = const_cast<stp_csr*>(hc);
+}  // extern "C"
+
+namespace {
+
+// Device workspace of one CGLS solve: r (rows, FP64), q (rows, FP32), p, s,
+// z, s_prev and the Jacobi diagonal (cols, FP32), the per-block partial sums
+// and the device-resident scalars.
+struct CglsWork {
+    double* r = nullptr;
+    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr;
+    double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *part4 = nullptr,
+           *part5 = nullptr;
+    stp::SolverScalars* sc = nullptr;
+};
+
+int cgls_check(const stp_csr* h, const double* b, double lam, int32_t max_iter, int32_t precond, void* ph,
+               const double* x, const char* who) {
     if (!h || !b || !x || max_iter < 0 || lam < 0 || precond < 0 || precond > 2)
-        return fail(STP_ERR_INVALID, "stp_cgls: bad argument");
-    if (precond == 2 && !precond_handle) return fail(STP_ERR_INVALID, "stp_cgls: learned preconditioner handle missing");
-    if (precond == 2 && stp::lpc_cols(precond_handle) != h->cols)
-        return fail(STP_ERR_STATE, "stp_cgls: learned preconditioner is not prepared for this matrix");
-    hipStream_t st = S(stream);
+        return fail(STP_ERR_INVALID, std::string(who) + ": bad argument");
+    if (precond == 2 && !ph) return fail(STP_ERR_INVALID, std::string(who) + ": learned preconditioner handle missing");
+    if (precond == 2 && stp::lpc_cols(ph) != h->cols)
+        return fail(STP_ERR_STATE, std::string(who) + ": learned preconditioner is not prepared for this matrix");
+    return STP_OK;
+}
+
+// The solve itself: kernel launches and host reads of the scalars only (the
+// solver layout, when A has a grid, must be built).
+int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_iter, int32_t precond,
+               void* precond_handle, double* x, double* history, stp_solve_info* info, hipStream_t st,
+               const CglsWork& w) {
     const int rows = h->rows, cols = h->cols;
     const bool lay = h->gn[0] > 0;
-    if (lay) {
-        const int rc = stp::build_solver_layout(h, st);
-        if (rc != STP_OK) return rc;
-    }
     const int* RP = lay ? h->lrp : h->rp;
     const int* CI = lay ? h->lci : h->ci;
     const float* VAL = lay ? h->lval : h->val;
@@ -353,89 +372,75 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     const int nbq = std::max(stp::spmv_blocks(rows, W), 1);
     const int nbs = std::max(stp::spmv_blocks(cols, WT), 1);
     const int nbc = std::max((cols + 255) / 256, 1);
-    STP_HIP(grow(&h->r, &h->c_r, rows));
-    STP_HIP(grow(&h->p, &h->c_p, cols));
-    STP_HIP(grow(&h->q, &h->c_q, rows));
-    STP_HIP(grow(&h->s, &h->c_s, cols));
-    STP_HIP(grow(&h->z, &h->c_z, cols));
-    STP_HIP(grow(&h->sp, &h->c_sp, cols));
-    STP_HIP(grow(&h->dinv, &h->c_dinv, cols));
-    STP_HIP(grow(&h->part0, &h->c_p0, nbq));
-    STP_HIP(grow(&h->part1, &h->c_p1, nbc));
-    STP_HIP(grow(&h->part2, &h->c_p2, nbs));
-    STP_HIP(grow(&h->part3, &h->c_p3, nbs));
-    STP_HIP(grow(&h->part4, &h->c_p4, nbc));
-    STP_HIP(grow(&h->part5, &h->c_p5, nbc));
-    STP_HIP(grow(&h->sc, &h->c_sc, 1));
     const bool flex = precond == 2;
     float* dinv = nullptr;
     if (precond == 1) {
-        STP_HIP(stp::launch_normal_diag(cols, TRP, TVAL, lam, h->dinv, st));
-        dinv = h->dinv;
+        STP_HIP(stp::launch_normal_diag(cols, TRP, TVAL, lam, w.dinv, st));
+        dinv = w.dinv;
     }
-    int* done = &h->sc->done;
+    int* done = &w.sc->done;
+    const size_t fw = sizeof(float) / 4, dw = sizeof(double) / 4;
 
     // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s
     auto start = [&](int set_s0) -> int {
-        STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
-                                     h->part3, nullptr, st));
+        STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3, nullptr,
+                                     st));
         if (flex) {
-            STP_HIP(hipMemsetAsync(h->sp, 0, sizeof(float) * cols, st));
-            int rc = stp::lpc_apply(precond_handle, h->s, h->z, nullptr, h->sp, h->part4, h->part5, st, perm);
+            STP_HIP(stp::launch_zero_words(w.sp, fw * cols, st));
+            int rc = stp::lpc_apply(precond_handle, w.s, w.z, nullptr, w.sp, w.part4, w.part5, st, perm);
             if (rc != STP_OK) return rc;
-            STP_HIP(stp::launch_cgls_init(h->sc, h->part4, nbc, h->part3, nbs, set_s0, st));
+            STP_HIP(stp::launch_cgls_init(w.sc, w.part4, nbc, w.part3, nbs, set_s0, st));
         } else {
-            STP_HIP(stp::launch_cgls_init(h->sc, h->part2, nbs, h->part3, nbs, set_s0, st));
+            STP_HIP(stp::launch_cgls_init(w.sc, w.part2, nbs, w.part3, nbs, set_s0, st));
         }
-        STP_HIP(hipMemsetAsync(h->p, 0, sizeof(float) * cols, st));
-        STP_HIP(hipMemsetAsync(&h->sc->beta, 0, sizeof(double), st));
-        STP_HIP(stp::launch_cgls_direction(h->sc, cols, h->z, h->p, h->part1, h->s, h->sp, st));
+        STP_HIP(stp::launch_zero_words(w.p, fw * cols, st));
+        STP_HIP(stp::launch_zero_words(&w.sc->beta, dw, st));
+        STP_HIP(stp::launch_cgls_direction(w.sc, cols, w.z, w.p, w.part1, w.s, w.sp, st));
         return STP_OK;
     };
 
-    STP_HIP(hipMemsetAsync(x, 0, sizeof(double) * cols, st));
-    STP_HIP(hipMemsetAsync(h->sc, 0, sizeof(stp::SolverScalars), st));
-    STP_HIP(hipMemcpyAsync(h->r, b, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
+    STP_HIP(stp::launch_zero_words(x, dw * cols, st));
+    STP_HIP(stp::launch_zero_words(w.sc, sizeof(stp::SolverScalars) / 4, st));
+    STP_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
     int rc = start(1);
     if (rc != STP_OK) return rc;
     stp::SolverScalars hs;
     std::memset(&hs, 0, sizeof hs);
     int it = 0, replacements = 0;
     double true_rel = 0.0;
-    std::vector<double> ps(nbs);
+    double ps[stp::SPMV_MAX_BLOCKS];
     for (;;) {
         for (; it < max_iter; ++it) {
-            STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, h->p, h->q, h->part0, done, st));
-            STP_HIP(stp::launch_cgls_alpha(h->sc, h->part0, nbq, h->part1, nbc, lam, st));
-            STP_HIP(stp::launch_cgls_update(h->sc, cols, x, h->p, rows, h->r, h->q, st));
-            STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z,
-                                         h->part2, h->part3, done, st));
+            STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st));
+            STP_HIP(stp::launch_cgls_alpha(w.sc, w.part0, nbq, w.part1, nbc, lam, st));
+            STP_HIP(stp::launch_cgls_update(w.sc, cols, x, w.p, rows, w.r, w.q, st));
+            STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3,
+                                         done, st));
             if (flex) {
-                rc = stp::lpc_apply(precond_handle, h->s, h->z, done, h->sp, h->part4, h->part5, st, perm);
+                rc = stp::lpc_apply(precond_handle, w.s, w.z, done, w.sp, w.part4, w.part5, st, perm);
                 if (rc != STP_OK) return rc;
-                STP_HIP(stp::launch_cgls_beta(h->sc, h->part4, nbc, h->part3, nbs, h->part5, nbc, tol, history, it,
-                                              st));
+                STP_HIP(stp::launch_cgls_beta(w.sc, w.part4, nbc, w.part3, nbs, w.part5, nbc, tol, history, it, st));
             } else {
-                STP_HIP(stp::launch_cgls_beta(h->sc, h->part2, nbs, h->part3, nbs, nullptr, 0, tol, history, it, st));
+                STP_HIP(stp::launch_cgls_beta(w.sc, w.part2, nbs, w.part3, nbs, nullptr, 0, tol, history, it, st));
             }
-            STP_HIP(stp::launch_cgls_direction(h->sc, cols, h->z, h->p, h->part1, h->s, flex ? h->sp : nullptr, st));
+            STP_HIP(stp::launch_cgls_direction(w.sc, cols, w.z, w.p, w.part1, w.s, flex ? w.sp : nullptr, st));
             if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
-                STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+                STP_HIP(hipMemcpyAsync(&hs, w.sc, sizeof hs, hipMemcpyDeviceToHost, st));
                 STP_HIP(hipStreamSynchronize(st));
                 if (hs.done) break;
             }
         }
-        STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
+        STP_HIP(hipMemcpyAsync(&hs, w.sc, sizeof hs, hipMemcpyDeviceToHost, st));
         STP_HIP(hipStreamSynchronize(st));
         it = hs.iters;
         // true residual in FP64
-        STP_HIP(stp::residual(W, rows, RP, CI, VAL, x, b, h->r, st));
-        STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, h->r, x, lam, dinv, h->s, h->z, h->part2,
-                                     h->part3, nullptr, st));
-        STP_HIP(hipMemcpyAsync(ps.data(), h->part3, sizeof(double) * nbs, hipMemcpyDeviceToHost, st));
+        STP_HIP(stp::residual(W, rows, RP, CI, VAL, x, b, w.r, st));
+        STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3, nullptr,
+                                     st));
+        STP_HIP(hipMemcpyAsync(ps, w.part3, sizeof(double) * nbs, hipMemcpyDeviceToHost, st));
         STP_HIP(hipStreamSynchronize(st));
         double ss = 0.0;
-        for (double v : ps) ss += v;
+        for (int k = 0; k < nbs; ++k) ss += ps[k];
         true_rel = hs.s0 > 0 ? std::sqrt(ss) / hs.s0 : 0.0;
         if (hs.done != 1 || true_rel < tol || it >= max_iter || replacements >= 8) break;
         ++replacements;
@@ -451,6 +456,124 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
         info->replacements = replacements;
     }
     return STP_OK;
+}
+
+// ---- workspace of the no-allocation entry point: one static device array
+// per GPU (hipGetSymbolAddress gives each device's copy), carved into the
+// CglsWork buffers at 256-byte offsets
+constexpr int64_t NA_MAX_ROWS = int64_t(1) << 20;
+constexpr int64_t NA_MAX_COLS = int64_t(1) << 20;
+constexpr size_t na_align(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t NA_NBC = (NA_MAX_COLS + 255) / 256;
+constexpr size_t NA_OFF_SC = 0;
+constexpr size_t NA_OFF_P0 = NA_OFF_SC + na_align(sizeof(stp::SolverScalars));
+constexpr size_t NA_OFF_P2 = NA_OFF_P0 + na_align(8 * stp::SPMV_MAX_BLOCKS);
+constexpr size_t NA_OFF_P3 = NA_OFF_P2 + na_align(8 * stp::SPMV_MAX_BLOCKS);
+constexpr size_t NA_OFF_P1 = NA_OFF_P3 + na_align(8 * stp::SPMV_MAX_BLOCKS);
+constexpr size_t NA_OFF_P4 = NA_OFF_P1 + na_align(8 * NA_NBC);
+constexpr size_t NA_OFF_P5 = NA_OFF_P4 + na_align(8 * NA_NBC);
+constexpr size_t NA_OFF_R = NA_OFF_P5 + na_align(8 * NA_NBC);
+constexpr size_t NA_OFF_Q = NA_OFF_R + na_align(8 * NA_MAX_ROWS);
+constexpr size_t NA_OFF_P = NA_OFF_Q + na_align(4 * NA_MAX_ROWS);
+constexpr size_t NA_OFF_S = NA_OFF_P + na_align(4 * NA_MAX_COLS);
+constexpr size_t NA_OFF_Z = NA_OFF_S + na_align(4 * NA_MAX_COLS);
+constexpr size_t NA_OFF_SP = NA_OFF_Z + na_align(4 * NA_MAX_COLS);
+constexpr size_t NA_OFF_DINV = NA_OFF_SP + na_align(4 * NA_MAX_COLS);
+constexpr size_t NA_BYTES = NA_OFF_DINV + na_align(4 * NA_MAX_COLS);   // 32.2 MiB
+constexpr int NA_MAX_DEVICES = 64;
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[NA_MAX_DEVICES] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+// Device address of this GPU's copy of `scratch` (looked up once per device,
+// with that device current).
+static int scratch_address(unsigned char** out) {
+    int dev = 0;
+    STP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) return fail(STP_ERR_INVALID, "stp_cgls_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "stp_cgls_noalloc: hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+    return STP_OK;
+}
+
+extern "C" {
+
+int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t max_iter, int32_t precond,
+             void* precond_handle, double* x, double* history, stp_solve_info* info, void* stream) {
+    stp_csr* h = const_cast<stp_csr*>(hc);
+    int rc = cgls_check(h, b, lam, max_iter, precond, precond_handle, x, "stp_cgls");
+    if (rc != STP_OK) return rc;
+    hipStream_t st = S(stream);
+    const int rows = h->rows, cols = h->cols;
+    if (h->gn[0] > 0) {
+        rc = stp::build_solver_layout(h, st);
+        if (rc != STP_OK) return rc;
+    }
+    const int W = h->gn[0] > 0 ? h->lW : h->W, WT = h->gn[0] > 0 ? h->lWT : h->WT;
+    const int nbq = std::max(stp::spmv_blocks(rows, W), 1);
+    const int nbs = std::max(stp::spmv_blocks(cols, WT), 1);
+    const int nbc = std::max((cols + 255) / 256, 1);
+    STP_HIP(grow(&h->r, &h->c_r, rows));
+    STP_HIP(grow(&h->p, &h->c_p, cols));
+    STP_HIP(grow(&h->q, &h->c_q, rows));
+    STP_HIP(grow(&h->s, &h->c_s, cols));
+    STP_HIP(grow(&h->z, &h->c_z, cols));
+    STP_HIP(grow(&h->sp, &h->c_sp, cols));
+    STP_HIP(grow(&h->dinv, &h->c_dinv, cols));
+    STP_HIP(grow(&h->part0, &h->c_p0, nbq));
+    STP_HIP(grow(&h->part1, &h->c_p1, nbc));
+    STP_HIP(grow(&h->part2, &h->c_p2, nbs));
+    STP_HIP(grow(&h->part3, &h->c_p3, nbs));
+    STP_HIP(grow(&h->part4, &h->c_p4, nbc));
+    STP_HIP(grow(&h->part5, &h->c_p5, nbc));
+    STP_HIP(grow(&h->sc, &h->c_sc, 1));
+    CglsWork w;
+    w.r = h->r; w.p = h->p; w.q = h->q; w.s = h->s; w.z = h->z; w.sp = h->sp; w.dinv = h->dinv;
+    w.part0 = h->part0; w.part1 = h->part1; w.part2 = h->part2; w.part3 = h->part3; w.part4 = h->part4;
+    w.part5 = h->part5; w.sc = h->sc;
+    return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, st, w);
+}
+
+int stp_cgls_noalloc(const stp_csr* hc, const double* b, double lam, double tol, int32_t max_iter, int32_t precond,
+                     void* precond_handle, double* x, double* history, stp_solve_info* info, void* stream) {
+    stp_csr* h = const_cast<stp_csr*>(hc);
+    int rc = cgls_check(h, b, lam, max_iter, precond, precond_handle, x, "stp_cgls_noalloc");
+    if (rc != STP_OK) return rc;
+    if (h->rows > NA_MAX_ROWS || h->cols > NA_MAX_COLS)
+        return fail(STP_ERR_INVALID, "stp_cgls_noalloc: at most 1048576 rows and 1048576 columns (its static "
+                                     "workspace); use stp_cgls for larger systems");
+    if (h->gn[0] > 0 && !h->lay_ready)
+        return fail(STP_ERR_STATE, "stp_cgls_noalloc: the matrix has a cell grid but no solver layout yet; call "
+                                   "stp_csr_prepare_layout first (it allocates)");
+    unsigned char* base = nullptr;
+    rc = scratch_address(&base);
+    if (rc != STP_OK) return rc;
+    CglsWork w;
+    w.sc = reinterpret_cast<stp::SolverScalars*>(base + NA_OFF_SC);
+    w.part0 = reinterpret_cast<double*>(base + NA_OFF_P0);
+    w.part1 = reinterpret_cast<double*>(base + NA_OFF_P1);
+    w.part2 = reinterpret_cast<double*>(base + NA_OFF_P2);
+    w.part3 = reinterpret_cast<double*>(base + NA_OFF_P3);
+    w.part4 = reinterpret_cast<double*>(base + NA_OFF_P4);
+    w.part5 = reinterpret_cast<double*>(base + NA_OFF_P5);
+    w.r = reinterpret_cast<double*>(base + NA_OFF_R);
+    w.q = reinterpret_cast<float*>(base + NA_OFF_Q);
+    w.p = reinterpret_cast<float*>(base + NA_OFF_P);
+    w.s = reinterpret_cast<float*>(base + NA_OFF_S);
+    w.z = reinterpret_cast<float*>(base + NA_OFF_Z);
+    w.sp = reinterpret_cast<float*>(base + NA_OFF_SP);
+    w.dinv = reinterpret_cast<float*>(base + NA_OFF_DINV);
+    return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, S(stream), w);
 }
 
 }  // extern "C"

@@ -11,6 +11,10 @@ struct SolverScalars {
     int iters;
 };
 
+// upper bound of the grid (and so of the per-block partial sums) of the SpMV
+// and normal-residual kernels: they stride over the rows
+constexpr int SPMV_MAX_BLOCKS = 2048;
+
 int spmv_lanes(double mean_nnz, int transpose);
 int spmv_lanes_layout(double mean_nnz, int transpose);
 int spmv_blocks(int rows, int W);
@@ -49,6 +53,8 @@ hipError_t launch_flex_dots(const int* done, int n, const float* z, const float*
 hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss, int set_s0,
                             hipStream_t st);
 hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st);
+// zero n 32-bit words with a kernel (no memset node when a caller captures)
+hipError_t launch_zero_words(void* p, size_t nwords, hipStream_t st);
 hipError_t launch_normal_diag(int n, const int* rp, const float* val, double lam, float* dinv, hipStream_t st);
 hipError_t launch_diag_inv(int n, const int* rp, const int* ci, const float* val, double* dinv, hipStream_t st);
 hipError_t launch_diag_inv_f64v(int n, const int* rp, const int* ci, const double* val, double* dinv,

@@ -25,7 +25,7 @@ namespace stp {
 // Rows are processed by groups of W lanes (RPB = 256 / W rows per block per
 // pass); blocks stride over the rows, so the grid is bounded (<= MAX_BLOCKS)
 // and each block leaves one FP64 partial per reduced quantity.
-constexpr int MAX_BLOCKS = 2048;
+constexpr int MAX_BLOCKS = SPMV_MAX_BLOCKS;
 
 // Each lane takes entries b+lane, b+lane+W, ... and accumulates them in that
 // order.  The row is consumed in chunks of U*W entries (U*W = 128): all U
@@ -356,6 +356,10 @@ __global__ void cgls_init_kernel(SolverScalars* sc, const double* psz, int nsz, 
         else sc->done = 0;
         sc->rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
     }
+}
+
+__global__ void zero_words_kernel(uint32_t* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 __global__ void copy_f32_kernel(int n, const float* a, float* b) {
@@ -754,6 +758,13 @@ hipError_t launch_flex_dots(const int* done, int n, const float* z, const float*
 hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss, int set_s0,
                             hipStream_t st) {
     hipLaunchKernelGGL(cgls_init_kernel, dim3(1), dim3(256), 0, st, sc, psz, nsz, pss, nss, set_s0);
+    return hipGetLastError();
+}
+hipError_t launch_zero_words(void* p, size_t nwords, hipStream_t st) {
+    if (nwords == 0) return hipSuccess;
+    size_t nb = (nwords + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)nb), dim3(256), 0, st, static_cast<uint32_t*>(p), nwords);
     return hipGetLastError();
 }
 hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st) {

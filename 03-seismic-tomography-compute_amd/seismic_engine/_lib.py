@@ -76,6 +76,7 @@ SIGNATURES: dict[str, list] = {
     "stp_csr_set_values_f64": [_vp, _vp, _vp],
     "stp_cg": [_vp, _vp, _i32, _f64, _i32, _vp, _vp, C.POINTER(SolveInfo), _vp],
     "stp_cgls": [_vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _vp, C.POINTER(SolveInfo), _vp],
+    "stp_cgls_noalloc": [_vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _vp, C.POINTER(SolveInfo), _vp],
     "stp_csr_assemble": [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64), _vp],
     "stp_csr64_fd7_triplets": [_i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _vp, _vp, _vp, _vp],
     "stp_csr64_scale": [_i64, _vp, _f64, _vp],

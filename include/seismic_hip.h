@@ -213,6 +213,20 @@ int stp_cg(const stp_csr* A, const double* b_dev, int32_t max_iter, double tol, 
 int stp_cgls(const stp_csr* A, const double* b_dev, double lambda, double tol, int32_t max_iter, int32_t precond,
              void* precond_handle, double* x_dev, double* history_dev, stp_solve_info* info, void* stream);
 
+/* stp_cgls without any device allocation: the same solve (same kernels, same
+ * results bit for bit) with its workspace in a static device array of the
+ * library (32.2 MiB per GPU, held until the process ends).  No counterpart in
+ * the reference: for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid hipMalloc).  Limits: at most 2^20 rows and 2^20
+ * columns (STP_ERR_INVALID beyond); a matrix with a cell grid must already
+ * have its solver layout (stp_csr_prepare_layout; STP_ERR_STATE otherwise);
+ * a learned preconditioner must be prepared.  Every call on a GPU shares that
+ * GPU's workspace: calls that overlap on one GPU (other streams or threads)
+ * race. */
+int stp_cgls_noalloc(const stp_csr* A, const double* b_dev, double lambda, double tol, int32_t max_iter,
+                     int32_t precond, void* precond_handle, double* x_dev, double* history_dev, stp_solve_info* info,
+                     void* stream);
+
 /* Canonical CSR assembly, CSRMatrix::build_from_triplets semantics
  * (csr_matrix.cpp:25-72) without its padding pass: triplets whose row or
  * column is out of range are dropped, rows are ordered by column, duplicate
