@@ -147,24 +147,6 @@ int stp_fd_plan_create(const stp_fd_config* c, stp_fd_plan** out) {
         p->cy[k] = (float)(a[k] / (c->dy * c->dy));
         p->cz[k] = (float)(a[k] / (c->dz * c->dz));
     }
-    // two-step kernel: order 8 with float4 columns (halo and nz multiples of 4)
-    {
-        // opt-in (STP_FD_PAIR=1): on MI355X the pair kernel is bound by VALU
-        // throughput and slower than two single steps at 512^3
-        const char* ev = std::getenv("STP_FD_PAIR");
-        const bool single = !(ev && std::atoi(ev) != 0);
-        if (const char* ev2 = std::getenv("STP_FD2_TILE")) {
-            const int v = std::atoi(ev2);
-            if (v >= 0 && v < stp::fd_step2_num_variants()) p->variant2 = v;
-        }
-        int tz2, ty2, lx2;
-        stp::fd_step2_tiling(p->variant2, &tz2, &ty2, &lx2);
-        p->ntz2 = (int)((c->nz + tz2 - 1) / tz2);
-        p->nty2 = (int)((c->ny + ty2 - 1) / ty2);
-        p->nxc2 = (int)((c->nx + lx2 - 1) / lx2);
-        p->pair_ok = !single && R == 4 && p->VW == 4 &&
-                     (int64_t)p->ntz2 * p->nty2 * p->nxc2 * c->nshots <= 0x7fffffff;
-    }
     hipError_t e = hipMalloc(&p->d_word, 16);
     if (e != hipSuccess) { delete p; return hip_fail(e, "hipMalloc"); }
     *out = p;
@@ -178,8 +160,6 @@ int stp_fd_plan_destroy(stp_fd_plan* p) {
     hipFree(p->d_rcv_id);
     hipFree(p->d_src);
     hipFree(p->d_word);
-    hipFree(p->d_rcv_plain);
-    hipFree(p->d_pair);
     delete p;
     return STP_OK;
 }
@@ -215,18 +195,16 @@ int stp_fd_set_receivers(stp_fd_plan* p, const int64_t* idx, int32_t n) {
         cell[k] = idx[r];
         id[k] = r;
     }
-    hipFree(p->d_blk_ptr); hipFree(p->d_rcv_cell); hipFree(p->d_rcv_id); hipFree(p->d_rcv_plain);
-    p->d_blk_ptr = nullptr; p->d_rcv_cell = nullptr; p->d_rcv_id = nullptr; p->d_rcv_plain = nullptr;
+    hipFree(p->d_blk_ptr); hipFree(p->d_rcv_cell); hipFree(p->d_rcv_id);
+    p->d_blk_ptr = nullptr; p->d_rcv_cell = nullptr; p->d_rcv_id = nullptr;
     p->nrcv = 0; p->nrcv_pts = 0;
     if (n == 0) return STP_OK;
     STP_HIP(hipMalloc(&p->d_blk_ptr, sizeof(int) * (ntile + 1)));
     STP_HIP(hipMalloc(&p->d_rcv_cell, sizeof(int64_t) * n));
     STP_HIP(hipMalloc(&p->d_rcv_id, sizeof(int) * n));
-    STP_HIP(hipMalloc(&p->d_rcv_plain, sizeof(int64_t) * n));
     STP_HIP(hipMemcpy(p->d_blk_ptr, cnt.data(), sizeof(int) * (ntile + 1), hipMemcpyHostToDevice));
     STP_HIP(hipMemcpy(p->d_rcv_cell, cell.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
     STP_HIP(hipMemcpy(p->d_rcv_id, id.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    STP_HIP(hipMemcpy(p->d_rcv_plain, idx, sizeof(int64_t) * n, hipMemcpyHostToDevice));
     p->nrcv = n;
     p->nrcv_pts = n;
     return STP_OK;
@@ -319,24 +297,6 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
         STP_HIP(stp::fd_launch_inject(cur, p->ncell, p->d_src, wavelet + t_offset, nw, c.nshots, S(st)));
     float* u = cur;
     float* v = prev;
-    // Pairs of steps go to the two-step kernel, which writes two fields that
-    // no workgroup of the launch reads: pair launches alternate between the
-    // caller's buffers and the plan's pair buffers, so an even number of them
-    // ends in the caller's buffers with the roles of an even step count.  An
-    // odd pair count runs its first pair as two single (in-place) steps; an
-    // odd step count ends with one single step, which swaps the roles as the
-    // single-step path does.
-    // (the two-step kernel implements the reference sponge only)
-    int npairs = p->pair_ok && p->sponge_mode == STP_SPONGE_REFERENCE ? nsteps / 2 : 0;
-    const int nsingle_first = (npairs % 2) ? 2 : 0;
-    if (npairs % 2) --npairs;
-    if (npairs > 0 && !p->d_pair) {
-        // halo cells are never written: they must read as 0 (Dirichlet), as
-        // in the caller's buffers
-        const size_t bytes = sizeof(float) * 2 * (size_t)c.nshots * (size_t)p->ncell;
-        STP_HIP(hipMalloc(&p->d_pair, bytes));
-        STP_HIP(hipMemsetAsync(p->d_pair, 0, bytes, S(st)));
-    }
     auto single = [&](int s) -> int {
         const int t = t_offset + s;
         a.ucur = u;
@@ -349,53 +309,21 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
         std::swap(u, v);
         return STP_OK;
     };
-    int s = 0;
     int rc;
-    for (; s < nsingle_first; ++s)
-        if ((rc = single(s)) != STP_OK) return rc;
-    if (npairs > 0) {
-        stp::FdStep2Args b;
-        std::memset(&b, 0, sizeof b);
-        b.coef = coef;
-        b.ncell = p->ncell;
-        b.nxh = p->nxh; b.nyh = p->nyh; b.nzh = p->nzh; b.hw = c.halo_width;
-        std::memcpy(b.cx, p->cx, sizeof b.cx);
-        std::memcpy(b.cy, p->cy, sizeof b.cy);
-        std::memcpy(b.cz, p->cz, sizeof b.cz);
-        b.sw = a.sw;
-        b.sigma_over_sw2 = a.sigma_over_sw2;
-        b.amp = a.amp;
-        b.ntz = p->ntz2; b.nty = p->nty2; b.nxc = p->nxc2; b.nshots = c.nshots;
-        b.src_idx = inject ? p->d_src : nullptr;
-        b.w_stride = nw;
-        b.traces = traces;
-        b.nrcv = traces ? p->nrcv : 0;
-        b.rcv_cell_plain = p->d_rcv_plain;
-        b.trace_stride = trace_stride;
-        float* f0 = p->d_pair;
-        float* f1 = p->d_pair + (size_t)c.nshots * (size_t)p->ncell;
-        for (int k = 0; k < npairs; ++k, s += 2) {
-            const int t = t_offset + s;
-            b.u = u;
-            b.p = v;
-            // outputs: the buffers this launch does not read
-            float* wo = (u == cur || u == prev) ? f0 : cur;
-            float* vo = (u == cur || u == prev) ? f1 : prev;
-            b.wout = wo;
-            b.vout = vo;
-            b.damp_prev = (s == 0) ? !prev_damped : 1;
-            b.w_mid = (inject && t + 1 < nw) ? wavelet + (t + 1) : nullptr;
-            b.w_next = (inject && s + 2 < nsteps && t + 2 < nw) ? wavelet + (t + 2) : nullptr;
-            b.status1 = status ? status + s : nullptr;
-            b.status2 = status ? status + s + 1 : nullptr;
-            b.t_col = s;
-            STP_HIP(stp::fd_launch_step2(b, p->variant2, S(st)));
-            u = wo;
-            v = vo;
-        }
-    }
+    int s = 0;
     for (; s < nsteps; ++s)
         if ((rc = single(s)) != STP_OK) return rc;
+    return STP_OK;
+}
+
+int stp_stream_3r1w_f32(int64_t n, const float* a, const float* b, const float* c, float* out, int32_t nontemporal,
+                        void* st) {
+    if (n < 0 || n % 4 != 0 || (n > 0 && (!a || !b || !c || !out)))
+        return fail(STP_ERR_INVALID, "stp_stream_3r1w_f32: n must be a non-negative multiple of 4");
+    const uintptr_t al = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
+                         reinterpret_cast<uintptr_t>(c) | reinterpret_cast<uintptr_t>(out);
+    if (al % 16 != 0) return fail(STP_ERR_INVALID, "stp_stream_3r1w_f32: arrays must be 16-byte aligned");
+    STP_HIP(stp::fd_launch_stream3(n, a, b, c, out, nontemporal != 0, S(st)));
     return STP_OK;
 }
 

@@ -58,39 +58,10 @@ struct FdStepArgs {
 // kernel is compiled separately for the field (1) and Laplacian (3) kinds.
 enum { FD_MODE_PLAIN = 0, FD_MODE_SNAPSHOT = 1, FD_MODE_ACCUMULATE = 2, FD_MODE_SNAPSHOT_LAP = 3 };
 
-// Arguments of one two-step launch (fd_step2.hip).
-struct FdStep2Args {
-    const float* u;           // u~^t, all shots (read only)
-    const float* p;           // u^{t-1} (read only)
-    const float* coef;
-    float* vout;              // u^{t+1} before the mid-pair injection
-    float* wout;              // u^{t+2} before the next injection
-    int64_t ncell;
-    int nxh, nyh, nzh, hw;
-    float cx[5], cy[5], cz[5];
-    int sw;
-    float sigma_over_sw2;
-    float amp;
-    int damp_prev;            // apply the sponge factor to p in step 1
-    int ntz, nty, nxc, nshots;
-    const int64_t* src_idx;   // [nshots] or null
-    const float* w_mid;       // &wavelet[t+1] (shot 0) or null
-    const float* w_next;      // &wavelet[t+2] (shot 0) or null
-    int64_t w_stride;
-    int* status1;             // step-status words of the two steps (or null)
-    int* status2;
-    float* traces;            // [shot][nrcv][trace_stride] or null
-    int nrcv;
-    const int64_t* rcv_cell_plain;   // receiver r -> with-halo cell
-    int64_t trace_stride;
-    int t_col;                // trace column of the first step of the pair
-};
-
-int fd_step2_num_variants();
-void fd_step2_tiling(int variant, int* tz, int* ty, int* lx);
-hipError_t fd_launch_step2(const FdStep2Args& a, int variant, hipStream_t st);
 
 int fd_num_variants();
+hipError_t fd_launch_stream3(int64_t n, const float* a, const float* b, const float* c, float* o, bool nt,
+                             hipStream_t st);
 void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx);
 hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, int mode, hipStream_t st);
 hipError_t fd_launch_inject(float* u, int64_t ncell, const int64_t* src_idx, const float* w, int64_t w_stride,

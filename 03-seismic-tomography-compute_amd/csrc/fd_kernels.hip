@@ -499,6 +499,39 @@ __global__ void fd_sponge_kernel(float* d, int nxh, int nyh, int nzh, int hw, in
     d[i] = sponge_of(dd, sw, sg);
 }
 
+// Streaming ceiling of the step's access pattern: out = a + b * c, three
+// reads and one write of FP32 per cell (the step's 16 algorithmic bytes)
+// with no stencil, one float4 per thread, optionally non-temporal.
+template <bool NT>
+__global__ void __launch_bounds__(256) stream3_kernel(const f4n* __restrict__ a, const f4n* __restrict__ b,
+                                                      const f4n* __restrict__ c, f4n* __restrict__ o, int64_t n4) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    if constexpr (NT) {
+        const f4n x = __builtin_nontemporal_load(a + i), y = __builtin_nontemporal_load(b + i),
+                  z = __builtin_nontemporal_load(c + i);
+        __builtin_nontemporal_store(x + y * z, o + i);
+    } else {
+        o[i] = a[i] + b[i] * c[i];
+    }
+}
+
+hipError_t fd_launch_stream3(int64_t n, const float* a, const float* b, const float* c, float* o, bool nt,
+                             hipStream_t st) {
+    const int64_t n4 = n / 4;
+    if (n4 == 0) return hipSuccess;
+    const unsigned nb = (unsigned)((n4 + 255) / 256);
+    if (nt)
+        hipLaunchKernelGGL(stream3_kernel<true>, dim3(nb), dim3(256), 0, st, reinterpret_cast<const f4n*>(a),
+                           reinterpret_cast<const f4n*>(b), reinterpret_cast<const f4n*>(c),
+                           reinterpret_cast<f4n*>(o), n4);
+    else
+        hipLaunchKernelGGL(stream3_kernel<false>, dim3(nb), dim3(256), 0, st, reinterpret_cast<const f4n*>(a),
+                           reinterpret_cast<const f4n*>(b), reinterpret_cast<const f4n*>(c),
+                           reinterpret_cast<f4n*>(o), n4);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ launchers
 template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL = false>
 static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStream_t st) {
@@ -521,46 +554,24 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
 }
 
 // Tile shapes (TZ, TY, TYR, LX, non-temporal P/C/N) compiled for the float4
-// path.  Variant 20 (128 x 16 cells, one row per thread, 512 threads, x chunks
-// of 256 planes) is the default: fastest measured at 512^3 (398.8 us/step
-// against 450.6 for 64-plane chunks, since each chunk re-reads 2R halo planes
-// of u^t).  The others are kept for tuning through STP_FD_TILE=<index>.
+// path; STP_FD_TILE=<index> selects one.  Entry 0 (128 x 16 cells, one row
+// per thread, 512 threads, x chunks of 256 planes) is the default: at 512^3
+// it gives 256 workgroups, one per CU, and measured fastest (398 us/step).
+// Measured and not kept: loads issued two or three planes ahead (415-418 us),
+// non-temporal loads of P and C (430 us) or store of N (402 us), two 256-thread
+// workgroups per CU (128 x 8 or 64 x 16 cells: 413-419 us), shorter x chunks,
+// which re-read 2R halo planes of u^t per chunk (64 planes: 451 us), and taller
+// tiles (128 x 32, 256 x 16: 454-1171 us, or spills).  The others stay for
+// tuning on other grids.
 #define STP_FD_VARIANTS(X)            \
-    X(0, 128, 8, 1, 64, false)        \
-    X(1, 64, 16, 1, 64, false)        \
-    X(2, 128, 8, 1, 128, false)       \
-    X(3, 128, 16, 2, 64, false)       \
-    X(4, 64, 16, 2, 128, false)       \
-    X(5, 128, 8, 1, 64, true)         \
-    X(6, 128, 16, 1, 64, false)       \
-    X(7, 128, 16, 1, 128, true)       \
-    X(8, 128, 8, 1, 128, true)        \
-    X(9, 256, 8, 1, 64, false)        \
-    X(10, 128, 12, 1, 96, false)      \
-    X(11, 64, 16, 2, 64, false)       \
-    X(12, 128, 32, 2, 64, false)      \
-    X(13, 128, 32, 4, 64, false)      \
-    X(14, 64, 32, 1, 64, false)       \
-    X(15, 128, 24, 2, 64, false)      \
-    X(16, 128, 32, 2, 128, false)     \
-    X(17, 128, 32, 2, 64, true)       \
-    X(18, 128, 16, 1, 128, false)     \
-    X(19, 128, 32, 2, 256, false)     \
-    X(20, 128, 16, 1, 256, false)     \
-    X(21, 64, 32, 2, 256, false)      \
-    X(22, 128, 32, 4, 128, false)     \
-    X(23, 256, 16, 2, 128, false)     \
-    X(24, 128, 16, 1, 256, true)      \
-    X(25, 128, 8, 1, 512, false)      \
-    X(26, 64, 16, 1, 512, false)      \
-    X(27, 128, 16, 1, 512, false)     \
-    X(28, 256, 16, 2, 256, false)     \
-    X(29, 256, 8, 1, 256, false)      \
-    X(30, 128, 32, 1, 128, false)     \
-    X(31, 64, 64, 2, 128, false)      \
-    X(32, 128, 24, 1, 192, false)
+    X(0, 128, 16, 1, 256, false)      \
+    X(1, 128, 16, 1, 128, false)      \
+    X(2, 128, 8, 1, 256, false)       \
+    X(3, 64, 16, 1, 256, false)       \
+    X(4, 128, 8, 1, 64, false)        \
+    X(5, 128, 16, 1, 256, true)
 
-int fd_num_variants() { return 33; }
+int fd_num_variants() { return 6; }
 
 // Fixed tiles of the other paths (must match fd_launch_step below).
 #define STP_FD_FIXED(X)                 \

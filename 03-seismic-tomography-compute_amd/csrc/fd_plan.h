@@ -7,7 +7,7 @@
 
 struct stp_fd_plan {
     stp_fd_config cfg;
-    int R = 0, VW = 1, variant = 20, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
+    int R = 0, VW = 1, variant = 0, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
     int nxh = 0, nyh = 0, nzh = 0;
     int64_t ncell = 0;
     float cx[5] = {0}, cy[5] = {0}, cz[5] = {0};
@@ -19,14 +19,8 @@ struct stp_fd_plan {
     int* d_rcv_id = nullptr;
     // sources
     int64_t* d_src = nullptr;
-    int64_t* d_rcv_plain = nullptr;   // receiver r -> cell (two-step path)
     // scratch
     unsigned int* d_word = nullptr;
-    // two-step path: tiling and the two output fields of a pair launch
-    bool pair_ok = false;
-    int variant2 = 0;                 // tile of the two-step kernel (STP_FD2_TILE)
-    int ntz2 = 0, nty2 = 0, nxc2 = 0;
-    float* d_pair = nullptr;          // 2 * nshots * ncell floats, allocated on first use
 };
 
 namespace stp {

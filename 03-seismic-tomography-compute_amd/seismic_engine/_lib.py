@@ -61,6 +61,7 @@ SIGNATURES: dict[str, list] = {
     "stp_fd_verify_cfl": [_vp, _f64, C.POINTER(_i32)],
     "stp_fd_propagate": [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp],
     "stp_fd_sponge_factors": [_vp, _vp, _vp],
+    "stp_stream_3r1w_f32": [_i64, _vp, _vp, _vp, _vp, _i32, _vp],
     "stp_count_nonfinite_f64": [_vp, _i64, C.POINTER(_i64), _vp],
     "stp_clamp_field_f64": [_vp, _i64, _f64, _f64, _vp],
     "stp_repair_field_nan_f64": [_vp, _i64, _i64, _i64, _i64, _i32, _f64, _vp],

@@ -82,6 +82,40 @@ def stream_copy_rate(dev, nbytes: int = 1 << 30, reps: int = 10) -> float:
     return 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
 
 
+def stream_3r1w_rate(dev, n: int, reps: int = 10) -> dict:
+    """The FD step's access pattern without the stencil (stp_stream_3r1w_f32:
+    out = a + b * c over n^3 FP32 cells, three reads and one write per cell),
+    plain and with non-temporal loads and stores: algorithmic GB/s of each,
+    timed with HIP events on the current stream.  The measured streaming
+    ceiling of the step's 16 bytes per cell."""
+    import ctypes as C
+    import torch
+    from seismic_engine._lib import call, ptr, stream_handle
+    m = n ** 3
+    a, b, c = (torch.full((m,), v, dtype=torch.float32, device=dev) for v in (1.0, 2.0, 0.5))
+    o = torch.empty_like(a)
+    out = {}
+    for nt in (0, 1):
+        def run():
+            call("stp_stream_3r1w_f32", C.c_int64(m), ptr(a), ptr(b), ptr(c), ptr(o), nt, stream_handle())
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream(dev)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            run()
+        e1.record(st)
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out["nontemporal" if nt else "plain"] = {"us": round(us, 2), "gb_s": round(16.0 * m / (us * 1e-6) / 1e9, 1)}
+    assert float(o[0]) == 2.0
+    del a, b, c, o
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(threads: int, vel_h, n: int, steps: int, sponge: str):
     """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
     on the benchmark workload itself: the same n^3 model, order 8, sponge,
@@ -675,6 +709,10 @@ def main() -> int:
     copy_gbs = stream_copy_rate(dev)
     result["roofline"]["stream_copy_gbs"] = round(copy_gbs, 1)
     result["roofline"]["frac_of_stream_copy"] = round(achieved / copy_gbs, 4)
+    s3 = stream_3r1w_rate(dev, n)
+    best3 = max(v["gb_s"] for v in s3.values())
+    result["roofline"]["stream_3r1w"] = s3
+    result["roofline"]["frac_of_stream_3r1w"] = round(achieved / best3, 4)
 
     if not args.no_rtm:
         log(f"RTM imaging: {n}^3, {nt} steps, recorded traces back-propagated")

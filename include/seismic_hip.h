@@ -127,6 +127,15 @@ int stp_fd_propagate(stp_fd_plan* plan, float* u_cur_dev, float* u_prev_dev, con
                      int32_t t_offset, float* traces_dev, int32_t trace_stride,
                      int32_t* step_status_dev, void* stream);
 
+/* Measurement kernel, no counterpart in the reference: out = a + b * c over n
+ * FP32 values (n a multiple of 4, device arrays 16-byte aligned), one float4
+ * per thread, with non-temporal loads and stores when nontemporal != 0.  It
+ * moves the FD step's algorithmic bytes (three reads and one write per cell)
+ * without the stencil: the measured streaming ceiling that bench.py reports
+ * the step kernel against. */
+int stp_stream_3r1w_f32(int64_t n, const float* a, const float* b, const float* c, float* out, int32_t nontemporal,
+                        void* stream);
+
 /* Sponge factor D per with-halo cell (1 outside the sponge, and in the halo),
  * apply_sponge_layer (wave_equation.cpp:262-297). */
 int stp_fd_sponge_factors(const stp_fd_plan* plan, float* d_dev, void* stream);
