@@ -68,6 +68,43 @@ AlignedVector<double> SparseOps::cgls(const CSRMatrix& A, const AlignedVector<do
     return x;
 }
 
+std::vector<double> SparseOps::smooth_gradient(const std::vector<double>& gradient, int64_t nx, int64_t ny,
+                                               int64_t nz, double sigma) {
+    if (nx < 0 || ny < 0 || nz < 0) throw std::invalid_argument("SparseOps::smooth_gradient: negative extent");
+    const int64_t n = nx * ny * nz;
+    std::vector<double> out(static_cast<size_t>(n), 0.0);
+    if (n == 0 || static_cast<int64_t>(gradient.size()) < n) return out;
+    dev::Buffer<double> din(n), dout(n), work(n);
+    dev::upload(din.get(), gradient.data(), n);
+    dev::check(stp_smooth_gradient_f64(nx, ny, nz, sigma, din.get(), dout.get(), work.get(), nullptr),
+               "SparseOps::smooth_gradient");
+    dev::download(out.data(), dout.get(), n);
+    return out;
+}
+
+std::vector<double> SparseOps::compute_directional_derivative(const CSRMatrix& A,
+                                                              const AlignedVector<double>& direction,
+                                                              const AlignedVector<double>& gradient) {
+    std::vector<double> out(gradient.size(), 0.0);
+    if (out.empty()) return out;
+    const AlignedVector<double> ad = A.spmv(direction);
+    const size_t m = std::min(out.size(), ad.size());
+    for (size_t i = 0; i < m; ++i) out[i] = ad[i];
+    return out;
+}
+
+std::vector<double> SparseOps::gaussian_filter_3d(int64_t nx, int64_t ny, int64_t nz, double sigma) {
+    if (nx < 0 || ny < 0 || nz < 0) throw std::invalid_argument("SparseOps::gaussian_filter_3d: negative extent");
+    const int64_t n = nx * ny * nz;
+    std::vector<double> out(static_cast<size_t>(n), 0.0);
+    if (n == 0) return out;
+    dev::Buffer<double> dout(n), work(1024);
+    dev::check(stp_gaussian_filter_3d_f64(nx, ny, nz, sigma, dout.get(), work.get(), nullptr),
+               "SparseOps::gaussian_filter_3d");
+    dev::download(out.data(), dout.get(), n);
+    return out;
+}
+
 double SparseOps::dot_product(const AlignedVector<double>& a, const AlignedVector<double>& b) {
     const int64_t n = static_cast<int64_t>(std::min(a.size(), b.size()));
     if (n == 0) return 0.0;

@@ -326,25 +326,39 @@ void WaveEquationSolver::apply_sponge_layer() {
 }
 
 void WaveEquationSolver::save_snapshot(int) {
-    sync_to_host();
     const auto& l = grid_.local();
     const int hw = grid_.params().halo_width;
-    field_.snapshots.reserve(field_.snapshots.size() + static_cast<size_t>(l.total_cells));
-    for (int64_t ix = 0; ix < l.nx_local; ++ix)
-        for (int64_t iy = 0; iy < l.ny_local; ++iy)
-            for (int64_t iz = 0; iz < l.nz_local; ++iz) {
-                const double v = field_.current[static_cast<size_t>(
-                    grid_.local_index_with_halo(ix + hw, iy + hw, iz + hw))];
-                field_.snapshots.push_back(std::isfinite(v) ? v : 0.0);
-            }
+    const int64_t n = l.total_cells;
+    const size_t base = field_.snapshots.size();
+    field_.snapshots.resize(base + static_cast<size_t>(n));
+    if (!host_stale_) {
+        // the host mirror is current (set through mutable_field, or no step since)
+        size_t k = base;
+        for (int64_t ix = 0; ix < l.nx_local; ++ix)
+            for (int64_t iy = 0; iy < l.ny_local; ++iy)
+                for (int64_t iz = 0; iz < l.nz_local; ++iz) {
+                    const double v = field_.current[static_cast<size_t>(
+                        grid_.local_index_with_halo(ix + hw, iy + hw, iz + hw))];
+                    field_.snapshots[k++] = std::isfinite(v) ? v : 0.0;
+                }
+    } else {
+        // the device is ahead: gather the interior there, download only it
+        dev::Buffer<double> d(n);
+        const int64_t nxh = l.nx_local + 2 * hw, nyh = l.ny_local + 2 * hw, nzh = l.nz_local + 2 * hw;
+        dev::check(stp_field_interior_f64(cur_, nxh, nyh, nzh, hw, d.get(), nullptr), "save_snapshot");
+        dev::download(field_.snapshots.data() + base, d.get(), n);
+    }
     ++snapshots_captured_;
 }
 
 double WaveEquationSolver::compute_energy() const {
-    sync_to_host();
     double e = 0.0;
-    for (double v : field_.current)
-        if (std::isfinite(v)) e += v * v;
+    if (!host_stale_) {
+        for (double v : field_.current)
+            if (std::isfinite(v)) e += v * v;
+    } else {
+        dev::check(stp_field_energy_f32(cur_, ncell_, &e, nullptr), "compute_energy");
+    }
     return mpi_.allreduce_sum(e);
 }
 
@@ -388,9 +402,13 @@ PropagationResult WaveEquationSolver::run_rtm(const std::vector<double>& source_
     }
     const int nt = config_.nt;
     const int64_t nrcv = static_cast<int64_t>(receivers_.size());
-    if (nrcv == 0) throw std::invalid_argument("run_rtm: set_receivers() first");
-    if (static_cast<int64_t>(receiver_residuals.size()) < nrcv * nt)
-        throw std::invalid_argument("run_rtm: receiver_residuals must hold nt samples per receiver");
+    if (nrcv == 0) {
+        // the reference's own input: a with-halo residual field
+        if (static_cast<int64_t>(receiver_residuals.size()) >= ncell_)
+            return run_rtm_reference_flow(source_wavelet, receiver_residuals);
+        return r;   // neither form: not completed
+    }
+    if (static_cast<int64_t>(receiver_residuals.size()) < nrcv * nt) return r;
     rtm_engine_->reset_image();
     sync_to_device();
     refresh_coefficients();
@@ -413,6 +431,52 @@ PropagationResult WaveEquationSolver::run_rtm(const std::vector<double>& source_
     dev::download(h.data(), traces_dev_, nrcv * nt);
     traces_.assign(h.begin(), h.end());
     r.timesteps_completed = nt;
+    r.completed_successfully = true;
+    return r;
+}
+
+// The reference's run_rtm as written (wave_equation.cpp:621-688): the image
+// condition of the forward field with itself after every step (slices
+// captured every RTMConfig::snapshot_interval steps), then one term of the
+// final forward field with the residual field, then finalize_image.
+PropagationResult WaveEquationSolver::run_rtm_reference_flow(const std::vector<double>& source_wavelet,
+                                                             const std::vector<double>& residual_field) {
+    PropagationResult r{StepResult::OK, 0, 0, 0, false};
+    const auto& l = grid_.local();
+    const int hw = grid_.params().halo_width;
+    const int64_t nyh = l.ny_local + 2 * hw, nzh = l.nz_local + 2 * hw;
+    rtm_engine_->reset_image();
+    reset_field();
+    dev::Buffer<double> f64(ncell_), res(ncell_);
+    const int64_t bytes = static_cast<int64_t>(sizeof(double)) * ncell_;
+    auto current_f64 = [&]() {
+        dev::check(stp_memset_d(f64.get(), 0, bytes, nullptr), "run_rtm");
+        dev::check(stp_accumulate_f32_f64(ncell_, cur_, f64.get(), nullptr), "run_rtm");
+    };
+    const int si = std::max(1, rtm_cfg_.snapshot_interval);
+    for (int t = 0; t < config_.nt; ++t) {
+        if (t < static_cast<int>(source_wavelet.size()))
+            inject_source(l.nx_local / 2, l.ny_local / 2, l.nz_local / 2, {source_wavelet[static_cast<size_t>(t)]});
+        const StepResult sr = step_forward(t);
+        if (sr == StepResult::FATAL_DIVERGENCE) {
+            r.status = sr;
+            r.timesteps_completed = t;
+            return r;
+        }
+        if (sr == StepResult::NAN_DETECTED_AND_REPAIRED) r.status = sr;
+        current_f64();
+        rtm_engine_->apply_image_condition_device(f64.get(), f64.get(), nyh, nzh, t);
+        if (t % si == 0 && rtm_engine_->capturing()) {
+            const std::vector<double>& cur = field().current;
+            rtm_engine_->extract_slices(cur.data(), cur.data(), nullptr, nyh, nzh, t);
+        }
+        ++r.timesteps_completed;
+    }
+    dev::upload(res.get(), residual_field.data(), ncell_);
+    current_f64();
+    rtm_engine_->apply_image_condition_device(f64.get(), res.get(), nyh, nzh, config_.nt);
+    rtm_engine_->finalize_image();
+    r.total_cells_repaired = ft_.stats().cells_repaired;
     r.completed_successfully = true;
     return r;
 }

@@ -335,6 +335,21 @@ int stp_fd_sponge_factors(const stp_fd_plan* p, float* d, void* st) {
     return STP_OK;
 }
 
+// ------------------------------------------------------------------ field diagnostics
+int stp_field_energy_f32(const float* u, int64_t n, double* out, void* st) {
+    if (!out || n < 0 || (n > 0 && !u)) return fail(STP_ERR_INVALID, "stp_field_energy_f32: bad argument");
+    STP_HIP(stp::field_energy_f32(u, n, out, S(st)));
+    return STP_OK;
+}
+
+int stp_field_interior_f64(const float* u, int64_t nxh, int64_t nyh, int64_t nzh, int32_t hw, double* out,
+                           void* st) {
+    if (!u || !out || hw < 0 || nxh < 2 * hw || nyh < 2 * hw || nzh < 2 * hw)
+        return fail(STP_ERR_INVALID, "stp_field_interior_f64: bad argument");
+    STP_HIP(stp::field_interior_f64(u, nxh, nyh, nzh, hw, out, S(st)));
+    return STP_OK;
+}
+
 // ------------------------------------------------------------------ health
 int stp_count_nonfinite_f64(const double* d, int64_t n, int64_t* out, void* st) {
     if (!out || n < 0 || (n > 0 && !d)) return fail(STP_ERR_INVALID, "stp_count_nonfinite_f64: bad argument");

@@ -94,8 +94,9 @@ int stp_fd_adjoint(stp_fd_plan* p, const float* coef, const float* wavelet, int3
                    const float* obs, const float* inj_traces, int32_t inj_scale_c, int32_t snap_lap,
                    int32_t checkpoint_interval, float* acc_out, float* acc_ill, float* traces_out, double* misfit,
                    void* stream) {
-    if (!p || !coef || !acc_out || nt < 1 || nw < 0)
+    if (!p || !coef || nt < 1 || nw < 0 || (!acc_out && !obs))
         return fail(STP_ERR_INVALID, "stp_fd_adjoint: bad argument");
+    const bool misfit_only = !acc_out;   // forward pass and misfit, no reverse pass
     if ((obs || inj_traces || traces_out) && p->nrcv == 0)
         return fail(STP_ERR_STATE, "stp_fd_adjoint: traces given but no receivers set");
     if (obs && inj_traces) return fail(STP_ERR_INVALID, "stp_fd_adjoint: give either obs or inj_traces");
@@ -117,9 +118,11 @@ int stp_fd_adjoint(stp_fd_plan* p, const float* coef, const float* wavelet, int3
         STP_HIP(hipMalloc(&res.p, sizeof(float) * ntr));
         STP_HIP(hipMalloc(&part.p, sizeof(double) * 1024));
     }
-    STP_HIP(hipMalloc(&work.p, sizeof(float) * field * 4));
-    STP_HIP(hipMalloc(&ckpt.p, sizeof(float) * field * 2 * nck));
-    STP_HIP(hipMalloc(&seg.p, sizeof(float) * field * K));
+    STP_HIP(hipMalloc(&work.p, sizeof(float) * field * (misfit_only ? 2 : 4)));
+    if (!misfit_only) {
+        STP_HIP(hipMalloc(&ckpt.p, sizeof(float) * field * 2 * nck));
+        STP_HIP(hipMalloc(&seg.p, sizeof(float) * field * K));
+    }
     float* w0 = static_cast<float*>(work.p);
     float* fu = w0;
     float* fv = w0 + field;
@@ -132,8 +135,11 @@ int stp_fd_adjoint(stp_fd_plan* p, const float* coef, const float* wavelet, int3
     STP_HIP(hipMemsetAsync(w0, 0, sizeof(float) * field * 2, st));
     for (int c = 0; c < nck; ++c) {
         const int t0 = c * K, len = std::min(K, nt - t0);
-        STP_HIP(hipMemcpyAsync(ck + (2 * c) * field, fu, sizeof(float) * field, hipMemcpyDeviceToDevice, st));
-        STP_HIP(hipMemcpyAsync(ck + (2 * c + 1) * field, fv, sizeof(float) * field, hipMemcpyDeviceToDevice, st));
+        if (!misfit_only) {
+            STP_HIP(hipMemcpyAsync(ck + (2 * c) * field, fu, sizeof(float) * field, hipMemcpyDeviceToDevice, st));
+            STP_HIP(hipMemcpyAsync(ck + (2 * c + 1) * field, fv, sizeof(float) * field, hipMemcpyDeviceToDevice,
+                                   st));
+        }
         int rc = forward_steps(p, &fu, &fv, coef, len, 0, wavelet, nw, t0, traces_out ? traces_out + t0 : nullptr, nt,
                                nullptr, 0, st);
         if (rc != STP_OK) return rc;
@@ -153,6 +159,8 @@ int stp_fd_adjoint(stp_fd_plan* p, const float* coef, const float* wavelet, int3
         if (misfit) *misfit = 0.5 * j;
         inj_traces = static_cast<const float*>(res.p);
     }
+
+    if (misfit_only) return STP_OK;   // the misfit read above synchronised the stream
 
     // reverse pass
     STP_HIP(hipMemsetAsync(au, 0, sizeof(float) * field * 2, st));

@@ -67,6 +67,37 @@ __global__ void replace_nonfinite_kernel(double* d, int64_t n, double repl) {
         if (!__builtin_isfinite(d[i])) d[i] = repl;
 }
 
+// WaveEquationSolver::compute_energy (wave_equation.cpp:482-490): sum of u^2
+// over the finite values of an FP32 field, FP64 partial sums per block (each
+// product of two FP32 values is exact in FP64)
+__global__ void energy_f32_kernel(const float* u, int64_t n, double* part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = (double)u[i];
+        if (__builtin_isfinite(v)) s += v * v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// WaveEquationSolver::save_snapshot (wave_equation.cpp:463-480): the interior
+// of a with-halo FP32 field as FP64, x-major, non-finite values as 0
+__global__ void interior_f64_kernel(const float* u, int64_t nxh, int64_t nyh, int64_t nzh, int hw, double* out) {
+    const int64_t nx = nxh - 2 * hw, ny = nyh - 2 * hw, nz = nzh - 2 * hw;
+    const int64_t n = nx * ny * nz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ix = i / (ny * nz), iy = (i / nz) % ny, iz = i % nz;
+        const double v = (double)u[((ix + hw) * nyh + (iy + hw)) * nzh + (iz + hw)];
+        out[i] = __builtin_isfinite(v) ? v : 0.0;
+    }
+}
+
 static unsigned grid_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 8192) b = 8192;
@@ -136,6 +167,34 @@ hipError_t health_repair(double* d, int64_t count, int64_t nx, int64_t ny, int64
         }
     }
     hipLaunchKernelGGL(replace_nonfinite_kernel, dim3(grid_for(count)), dim3(256), 0, st, d, count, repl);
+    return hipGetLastError();
+}
+
+hipError_t field_energy_f32(const float* u, int64_t n, double* out_host, hipStream_t st) {
+    *out_host = 0.0;
+    if (n <= 0) return hipSuccess;
+    const unsigned nb = grid_for(n) > 1024 ? 1024 : grid_for(n);
+    double* part = nullptr;
+    hipError_t e = hipMalloc(&part, sizeof(double) * nb);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(energy_f32_kernel, dim3(nb), dim3(256), 0, st, u, n, part);
+    e = hipGetLastError();
+    double hp[1024];
+    if (e == hipSuccess) e = hipMemcpyAsync(hp, part, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(part);
+    if (e != hipSuccess) return e;
+    double sum = 0.0;
+    for (unsigned k = 0; k < nb; ++k) sum += hp[k];
+    *out_host = sum;
+    return hipSuccess;
+}
+
+hipError_t field_interior_f64(const float* u, int64_t nxh, int64_t nyh, int64_t nzh, int hw, double* out,
+                              hipStream_t st) {
+    const int64_t n = (nxh - 2 * hw) * (nyh - 2 * hw) * (nzh - 2 * hw);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(interior_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, u, nxh, nyh, nzh, hw, out);
     return hipGetLastError();
 }
 

@@ -188,6 +188,51 @@ hipError_t smooth_passes(int64_t nx, int64_t ny, int64_t nz, int r, const Weight
     return hipGetLastError();
 }
 
+// SparseOps::gaussian_filter_3d (sparse_ops.cpp:212-234): the Gaussian
+// exp(-|c|^2 / (2 sigma^2)) of every cell, c = (ix, iy, iz) - (nx, ny, nz) / 2.0,
+// normalised by the sum over the grid.  Pass 1 writes the values and one
+// partial sum per block; pass 2 has every block add the partials in the same
+// order and scale its cells.
+constexpr int GAUSS_BLOCKS = 1024;
+
+__global__ void __launch_bounds__(256) gauss3_values_kernel(int64_t nx, int64_t ny, int64_t nz, double s2,
+                                                            double* __restrict__ out, double* __restrict__ part) {
+    const int64_t n = nx * ny * nz;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t ix = i / (ny * nz), iy = (i / nz) % ny, iz = i % nz;
+        const double cx = (double)ix - (double)nx / 2.0, cy = (double)iy - (double)ny / 2.0,
+                     cz = (double)iz - (double)nz / 2.0;
+        const double v = exp(-(cx * cx + cy * cy + cz * cz) / s2);
+        out[i] = v;
+        acc += v;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) gauss3_scale_kernel(int64_t n, double* __restrict__ out,
+                                                           const double* __restrict__ part, int np) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < np; k += 256) acc += part[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const double inv = 1.0 / red[0];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] *= inv;
+}
+
 }  // namespace
 }  // namespace stp
 
@@ -211,6 +256,24 @@ int stp_smooth_gradient_f64(int64_t nx, int64_t ny, int64_t nz, double sigma, co
     hipStream_t st = S(stream);
     STP_HIP(r <= 8 ? stp::smooth_passes_fast(nx, ny, nz, r, wt, in, out, work, st)
                    : stp::smooth_passes<0>(nx, ny, nz, r, wt, in, out, work, st));
+    return STP_OK;
+}
+
+int stp_gaussian_filter_3d_f64(int64_t nx, int64_t ny, int64_t nz, double sigma, double* out, double* work,
+                               void* stream) {
+    if (nx < 0 || ny < 0 || nz < 0 || !(sigma > 0.0) || !std::isfinite(sigma))
+        return fail(STP_ERR_INVALID, "stp_gaussian_filter_3d_f64: bad argument");
+    const int64_t n = nx * ny * nz;
+    if (n == 0) return STP_OK;
+    if (!out || !work) return fail(STP_ERR_INVALID, "stp_gaussian_filter_3d_f64: null pointer");
+    int64_t nb = (n + 255) / 256;
+    if (nb > stp::GAUSS_BLOCKS) nb = stp::GAUSS_BLOCKS;
+    hipStream_t st = S(stream);
+    hipLaunchKernelGGL(stp::gauss3_values_kernel, dim3((unsigned)nb), dim3(256), 0, st, nx, ny, nz,
+                       2.0 * sigma * sigma, out, work);
+    STP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(stp::gauss3_scale_kernel, dim3((unsigned)nb), dim3(256), 0, st, n, out, work, (int)nb);
+    STP_HIP(hipGetLastError());
     return STP_OK;
 }
 

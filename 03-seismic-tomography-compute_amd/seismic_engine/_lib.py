@@ -62,6 +62,8 @@ SIGNATURES: dict[str, list] = {
     "stp_fd_propagate": [_vp, _vp, _vp, _vp, _i32, _i32, _vp, _i32, _i32, _vp, _i32, _vp, _vp],
     "stp_fd_sponge_factors": [_vp, _vp, _vp],
     "stp_stream_3r1w_f32": [_i64, _vp, _vp, _vp, _vp, _i32, _vp],
+    "stp_field_energy_f32": [_vp, _i64, C.POINTER(_f64), _vp],
+    "stp_field_interior_f64": [_vp, _i64, _i64, _i64, _i32, _vp, _vp],
     "stp_count_nonfinite_f64": [_vp, _i64, C.POINTER(_i64), _vp],
     "stp_clamp_field_f64": [_vp, _i64, _f64, _f64, _vp],
     "stp_repair_field_nan_f64": [_vp, _i64, _i64, _i64, _i64, _i32, _f64, _vp],
@@ -89,6 +91,7 @@ SIGNATURES: dict[str, list] = {
     "stp_rtm_finalize_f64": [_i64, _vp, _vp, _vp, _vp],
     "stp_accumulate_f32_f64": [_i64, _vp, _vp, _vp],
     "stp_smooth_gradient_f64": [_i64, _i64, _i64, _f64, _vp, _vp, _vp, _vp],
+    "stp_gaussian_filter_3d_f64": [_i64, _i64, _i64, _f64, _vp, _vp, _vp],
     "stp_adjoint_gradient_f64": [_i64, _vp, _vp, _vp, _f64, _vp, _vp],
     "stp_lpc_create": [C.c_uint64, C.POINTER(_vp)],
     "stp_lpc_destroy": [_vp],

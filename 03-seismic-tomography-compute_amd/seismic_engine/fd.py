@@ -219,6 +219,26 @@ class FDPropagator:
              C.byref(mis), stream_handle(self.device))
         return {"acc": acc, "ill": ill, "traces": tr, "misfit": mis.value if obs is not None else None}
 
+    def misfit(self, nt: int, wavelet, obs):
+        """J = 1/2 sum (d - obs)^2 and the modelled traces from one forward
+        pass from rest (no checkpoints, no reverse pass); the field state is
+        not modified."""
+        torch = _torch()
+        if not self._have_src:
+            raise RuntimeError("set_sources() before misfit()")
+        w = torch.as_tensor(np.asarray(wavelet) if not torch.is_tensor(wavelet) else wavelet,
+                            dtype=torch.float32).to(self.device)
+        if w.dim() == 1:
+            w = w.expand(self.nshots, -1)
+        w = w.contiguous()
+        o = torch.as_tensor(np.asarray(obs) if not torch.is_tensor(obs) else obs, dtype=torch.float32)
+        o = o.to(self.device).reshape(self.nshots, self.nrcv, nt).contiguous()
+        tr = torch.empty((self.nshots, self.nrcv, nt), dtype=torch.float32, device=self.device)
+        mis = C.c_double(float("nan"))
+        call("stp_fd_adjoint", self._plan, ptr(self.coef), ptr(w), int(w.shape[1]), int(nt), ptr(o), None, 0, 0, 0,
+             None, None, ptr(tr), C.byref(mis), stream_handle(self.device))
+        return mis.value, tr
+
     def misfit_gradient(self, nt: int, wavelet, obs, checkpoint_interval: int = 0):
         """J = 1/2 sum (d - obs)^2 and dJ/dv (nshots summed), FP32 on the GPU.
         dJ/dv = 2 acc / v with acc = sum_k phi^k L u~^{k-1} (adjoint recursion)."""

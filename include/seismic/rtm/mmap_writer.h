@@ -19,6 +19,8 @@
 // and frame_count is the number of frames between 128 and write_offset.
 #pragma once
 
+#include <cstddef>
+
 #include <atomic>
 #include <cstdint>
 #include <string>
@@ -34,7 +36,9 @@ constexpr int64_t DEFAULT_MMAP_MAX_BYTES = 4LL * 1024LL * 1024LL * 1024LL;
 constexpr int64_t MMAP_FILE_MAGIC = 0x52544D534E4150LL;
 constexpr int64_t MMAP_HEADER_BYTES = 128;
 
-#pragma pack(push, 1)
+// every field is 8 bytes at an 8-byte offset, so the natural layout is the
+// packed one and the atomics on frame_count / write_offset / ring_start are
+// aligned (the mapping starts on a page)
 struct MmapFileHeader {
     int64_t magic;
     int64_t header_size;
@@ -45,8 +49,9 @@ struct MmapFileHeader {
     int64_t ring_capacity;
     int64_t reserved[8];
 };
-#pragma pack(pop)
 static_assert(sizeof(MmapFileHeader) == 120, "MmapFileHeader layout");
+static_assert(offsetof(MmapFileHeader, frame_count) == 24 && offsetof(MmapFileHeader, ring_start) == 40,
+              "MmapFileHeader layout");
 
 uint32_t slice_checksum(const uint8_t* data, size_t n);
 // serialize_slice / deserialize_slice of the reference's snapshot buffer;

@@ -80,6 +80,8 @@ public:
     };
     using SliceCaptureCallback = std::function<void(const CapturedSlice&)>;
     void set_slice_callback(SliceCaptureCallback cb) { slice_cb_ = std::move(cb); }
+    // New: a callback and slice specs are set (extract_slices has work to do)
+    bool capturing() const { return static_cast<bool>(slice_cb_) && !slice_specs_.empty(); }
     int64_t total_captured_slices() const { return slices_captured_.load(std::memory_order_acquire); }
 
     // device arrays (interior-indexed FP64)

@@ -10,13 +10,16 @@
 // Order: SolverConfig::fd_order (2 = the reference's 7-point stencil, 8 =
 // radius-4 star stencil; 0 = 8 when the halo width is >= 4, else 2).
 //
-// run_rtm is a true reverse-time migration: a checkpointed forward pass of
-// the source wavelet, then the receiver residuals (one trace of nt samples per
-// receiver set with set_receivers) propagated backwards with zero-lag
-// cross-correlation into rtm_engine()->result() (image += S R,
-// illumination += S S), then finalize_image.  The reference correlates the
-// forward field with itself; snapshot ring buffers and the mmap writer are
-// not part of this build.
+// run_rtm with receivers set (set_receivers) is a true reverse-time
+// migration: a checkpointed forward pass of the source wavelet, then the
+// receiver residuals (one trace of nt samples per receiver) propagated
+// backwards with zero-lag cross-correlation into rtm_engine()->result()
+// (image += S R, illumination += S S), then finalize_image.  With no
+// receivers and a with-halo residual field (n_h^3 values, the reference's
+// input) it follows the reference's flow: the forward field correlated with
+// itself after every step, then once with the residual field.  Inputs that
+// fit neither form return completed_successfully = false (no exception).
+// Snapshot ring buffers and the mmap drain thread are not part of this build.
 #pragma once
 
 #include <atomic>
@@ -138,6 +141,8 @@ public:
     const float* coefficients_device() const { return coef_; }
 
 private:
+    PropagationResult run_rtm_reference_flow(const std::vector<double>& source_wavelet,
+                                             const std::vector<double>& residual_field);
     const MpiContext& mpi_;
     const CartesianGrid& grid_;
     SolverConfig config_;

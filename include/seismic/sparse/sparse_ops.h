@@ -1,12 +1,12 @@
 // Sparse solvers (drop-in for the reference's src/cpp/sparse/sparse_ops.h:9-48).
 // CG and Jacobi-PCG keep the reference's signatures and stopping rules and
-// run on the GPU (stp_cg: FP64 vectors and dots, FP32 matrix values, x0 = 0).
+// run on the GPU (stp_cg: FP64 vectors, dots and matrix values, x0 = 0).
 // CGLS on the damped normal equations is new.  smooth_gradient,
-// gaussian_filter_3d and compute_directional_derivative are not part of this
-// build.
+// gaussian_filter_3d and compute_directional_derivative run on the GPU too.
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "csr_matrix.h"
 
@@ -38,6 +38,23 @@ public:
     static AlignedVector<double> cgls(const CSRMatrix& A, const AlignedVector<double>& b, double lambda,
                                       double tolerance, int max_iter, CglsPreconditioner precond,
                                       SolveInfo* info = nullptr, void* network = nullptr);
+
+    // Gaussian smoothing, radius (int)(3 sigma), weights renormalised over the
+    // in-range cells (sparse_ops.cpp:115-158); a gradient shorter than
+    // nx*ny*nz gives zeros.  sigma > 0, radius <= 64.
+    static std::vector<double> smooth_gradient(const std::vector<double>& gradient, int64_t nx, int64_t ny,
+                                               int64_t nz, double sigma);
+
+    // (A direction)[i] for i < gradient.size() (sparse_ops.cpp:160-170); the
+    // reference reads past A's rows when gradient is longer, here those
+    // entries are 0
+    static std::vector<double> compute_directional_derivative(const CSRMatrix& A,
+                                                              const AlignedVector<double>& direction,
+                                                              const AlignedVector<double>& gradient);
+
+    // the normalised Gaussian exp(-|c|^2 / (2 sigma^2)) of every cell of an
+    // nx x ny x nz grid, c = index - n / 2.0 (sparse_ops.cpp:212-234)
+    static std::vector<double> gaussian_filter_3d(int64_t nx, int64_t ny, int64_t nz, double sigma);
 
     // FP64 dot product over min(|a|, |b|) entries (sparse_ops.cpp:172-187)
     static double dot_product(const AlignedVector<double>& a, const AlignedVector<double>& b);

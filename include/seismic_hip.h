@@ -140,6 +140,16 @@ int stp_stream_3r1w_f32(int64_t n, const float* a, const float* b, const float* 
  * apply_sponge_layer (wave_equation.cpp:262-297). */
 int stp_fd_sponge_factors(const stp_fd_plan* plan, float* d_dev, void* stream);
 
+/* WaveEquationSolver::compute_energy (wave_equation.cpp:482-490): sum of u^2
+ * over the finite values of an FP32 device field of n cells (halo included),
+ * accumulated in FP64.  Synchronises. */
+int stp_field_energy_f32(const float* u_dev, int64_t n, double* energy_host, void* stream);
+/* WaveEquationSolver::save_snapshot (wave_equation.cpp:463-480): the interior
+ * (nxh-2hw) x (nyh-2hw) x (nzh-2hw) cells of a with-halo FP32 field as FP64,
+ * x-major, non-finite values as 0.  Asynchronous. */
+int stp_field_interior_f64(const float* u_dev, int64_t nxh, int64_t nyh, int64_t nzh, int32_t hw, double* out_dev,
+                           void* stream);
+
 /* ---------------------------------------------------------------- field health
  * GPU versions of FaultTolerance::count_nan_inf / clamp_field /
  * repair_field_nan (fault_tolerance.cpp:72-157), as bound by
@@ -261,6 +271,12 @@ int stp_dot_f64(int64_t n, const double* a_dev, const double* b_dev, double* out
  * (x-major), work is scratch.  out must not alias in.  Asynchronous. */
 int stp_smooth_gradient_f64(int64_t nx, int64_t ny, int64_t nz, double sigma, const double* in_dev, double* out_dev,
                             double* work_dev, void* stream);
+/* SparseOps::gaussian_filter_3d (sparse_ops.cpp:212-234): out[ix,iy,iz] =
+ * exp(-|c|^2 / (2 sigma^2)) / (sum over the grid), c = (ix, iy, iz) - n / 2.0
+ * per axis; out is nx*ny*nz FP64 (x-major), work holds 1024 FP64 partial
+ * sums.  Asynchronous. */
+int stp_gaussian_filter_3d_f64(int64_t nx, int64_t ny, int64_t nz, double sigma, double* out_dev, double* work_dev,
+                               void* stream);
 /* FP64/int64 CSR -> FP32/int32 arrays for stp_csr_create */
 int stp_csr64_to_f32(int64_t rows, int64_t nnz, const int64_t* row_ptr_dev, const int64_t* col_dev,
                      const double* val_dev, int32_t* row_ptr32_dev, int32_t* col32_dev, float* val32_dev,
@@ -280,7 +296,10 @@ int stp_csr64_to_f32(int64_t rows, int64_t nnz, const int64_t* row_ptr_dev, cons
  * dJ/dv = 2 acc_out / v for J = 1/2 sum (d - d_obs)^2 with e = d - d_obs).
  *   inj_traces_dev : [shot][nrcv][nt] injected at the receivers (may be NULL)
  *   inj_scale_c    : multiply the injection by C at the receiver cell
- *   acc_out_dev, acc_ill_dev (may be NULL): [shot][nx*ny*nz], interior-indexed
+ *   acc_out_dev    : [shot][nx*ny*nz], interior-indexed; NULL (with obs_dev
+ *                    given) runs the forward pass and the misfit only, with no
+ *                    checkpoints and no reverse pass
+ *   acc_ill_dev    : [shot][nx*ny*nz] (may be NULL)
  *   traces_out_dev : forward receiver traces [shot][nrcv][nt] (may be NULL)
  *   obs_dev        : observed traces [shot][nrcv][nt]; when given, the injected
  *                    traces are the residual e = d - obs of this call's forward

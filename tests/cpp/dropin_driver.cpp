@@ -10,8 +10,11 @@
 //   dropin_driver fd <dir>                 forward / step / adjoint propagation
 //   dropin_driver sparse <dir>             assembly, SpMV, CG, Jacobi-PCG, CGLS
 //   dropin_driver rtm <dir>                imaging condition, finalize, run_rtm
+//   dropin_driver smooth <dir>             smooth_gradient, gaussian_filter_3d
 //   dropin_driver health <dir>             count / clamp / repair of a field
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -142,12 +145,14 @@ int cmd_fd(const std::string& dir) {
     s.set_receivers(rcv);
     std::printf("order %d cfl %d\n", s.order(), s.verify_cfl() ? 1 : 0);
     const PropagationResult r = s.forward_propagate(w);
+    const double energy = s.compute_energy();   // before field(): reduced on the device
     print_result("forward", r);
     write(dir + "/cur.f64", s.field().current);
     write(dir + "/prev.f64", s.field().previous);
     write(dir + "/traces.f64", s.receiver_traces());
     write(dir + "/snapshots.f64", s.field().snapshots);
-    std::printf("snapshots %lld energy %.17g\n", (long long)s.snapshots_captured(), s.compute_energy());
+    std::printf("snapshots %lld energy %.17g host_energy %.17g\n", (long long)s.snapshots_captured(), energy,
+                s.compute_energy());
 
     // the same run by hand: inject at the centre, then step (wave_equation.cpp:118-125)
     WaveEquationSolver m(mpi, grid, cfg);
@@ -196,6 +201,9 @@ int cmd_sparse(const std::string& dir) {
     for (size_t i = 0; i < uv.size(); ++i) u[i] = uv[i];
     const AlignedVector<double> yt = A.spmv_transpose(u);
     write(dir + "/yt.f64", yt.data(), yt.size());
+    // (A x)[i] for the first rows - 5 entries of a gradient-length vector
+    AlignedVector<double> glen(static_cast<size_t>(rows - 5), 0.0);
+    write(dir + "/dd.f64", SparseOps::compute_directional_derivative(A, x, glen));
     std::printf("dot %.17g\n", SparseOps::dot_product(x, x));
 
     // square SPD system: the 7-point FD matrix shifted to be diagonally dominant
@@ -224,6 +232,22 @@ int cmd_sparse(const std::string& dir) {
     write(dir + "/cgls.f64", x3.data(), x3.size());
     std::printf("cgls iters %d converged %d true_rel %.3e\n", i3.iterations, i3.converged ? 1 : 0,
                 i3.true_rel_residual);
+    return 0;
+}
+
+// SparseOps::smooth_gradient and gaussian_filter_3d on the grid of
+// params.txt (no CSR matrix involved)
+int cmd_smooth(const std::string& dir) {
+    const int64_t nx = (int64_t)param(dir, "nx"), ny = (int64_t)param(dir, "ny"), nz = (int64_t)param(dir, "nz");
+    const double sigma = param(dir, "sigma");
+    const std::vector<double> g = read<double>(dir + "/g.f64");
+    write(dir + "/smooth.f64", SparseOps::smooth_gradient(g, nx, ny, nz, sigma));
+    write(dir + "/gauss.f64", SparseOps::gaussian_filter_3d(nx, ny, nz, sigma));
+    const std::vector<double> shortg(g.begin(), g.begin() + (std::ptrdiff_t)(g.size() / 2));
+    const std::vector<double> z = SparseOps::smooth_gradient(shortg, nx, ny, nz, sigma);
+    double zmax = 0.0;
+    for (double v : z) zmax = std::max(zmax, std::fabs(v));
+    std::printf("short_gradient size %zu max %.3g\n", z.size(), zmax);
     return 0;
 }
 
@@ -276,6 +300,18 @@ int cmd_rtm(const std::string& dir) {
     write(dir + "/rtm_image.f64", s.rtm_engine()->result().image);
     write(dir + "/rtm_illum.f64", s.rtm_engine()->result().illumination);
     write(dir + "/rtm_traces.f64", s.receiver_traces());
+
+    // the reference's own input: no receivers, a with-halo residual field
+    WaveEquationSolver s2(mpi, grid, cfg);
+    s2.set_velocity_field(vel.data(), (int64_t)vel.size());
+    s2.configure_rtm(rc);
+    const std::vector<double> rfield = read<double>(dir + "/resid_field.f64");
+    print_result("rtm_field", s2.run_rtm(w, rfield));
+    write(dir + "/rtmf_image.f64", s2.rtm_engine()->result().image);
+    write(dir + "/rtmf_illum.f64", s2.rtm_engine()->result().illumination);
+    write(dir + "/rtmf_refl.f64", s2.rtm_engine()->result().reflectivity);
+    // neither form: not completed, and no exception
+    print_result("rtm_short", s2.run_rtm(w, std::vector<double>(10, 0.0)));
     return 0;
 }
 
@@ -418,6 +454,7 @@ int main(int argc, char** argv) {
         if (cmd == "fd") return cmd_fd(arg);
         if (cmd == "sparse") return cmd_sparse(arg);
         if (cmd == "rtm") return cmd_rtm(arg);
+        if (cmd == "smooth") return cmd_smooth(arg);
         if (cmd == "health") return cmd_health(arg);
         if (cmd == "terrain") return cmd_terrain(arg);
         if (cmd == "mmap") return cmd_mmap(arg);

@@ -129,3 +129,26 @@ def test_autograd_function(cuda):
     J2, g2, _ = fd.misfit_gradient(NT, w, obs)
     assert abs(J.item() - J2) <= 1e-12 * J2
     assert torch.equal(v.grad, g2)
+
+
+def test_misfit_without_gradient_is_one_forward_pass(cuda):
+    """waveform_misfit without a gradient requested runs the forward pass only
+    (stp_fd_adjoint with no accumulator): the same misfit and traces as the
+    gradient call, and the same misfit as the FP64 oracle."""
+    import torch
+    from seismic_engine.autograd import waveform_misfit
+    fd, v_true, v0, w, rcv = _setup(cuda)
+    obs = _obs(fd, v_true, w)
+    v = torch.tensor(v0, dtype=torch.float32, device=cuda)
+    with torch.no_grad():
+        J0, tr0 = waveform_misfit(v.clone().requires_grad_(True), fd, w, obs, NT)
+    J1, tr1 = waveform_misfit(v, fd, w, obs, NT)
+    vg = v.clone().requires_grad_(True)
+    J2, tr2 = waveform_misfit(vg, fd, w, obs, NT)
+    J2.backward()
+    assert J0.item() == J1.item() == J2.item()
+    assert torch.equal(tr0, tr2) and torch.equal(tr1, tr2)
+    assert vg.grad is not None and float(vg.grad.abs().max()) > 0
+    c = ofd.cfg(N, hw=HW, order=8, sponge_width=6)
+    Jo = oad.misfit(c, synthetic.pad_halo(v0, HW), w, NT, fd.index(12, 10, 3), rcv, obs[0].cpu().numpy().astype(np.float64))
+    assert abs(J1.item() - Jo) / Jo < 1e-5

@@ -54,7 +54,15 @@ def test_solver_forward_step_adjoint_match_oracle(tmp_path, cuda):
     # snapshots after steps 0, 10, 20, 30 (t % snapshot_interval == 0)
     snaps = f64(tmp_path, "snapshots.f64").reshape(4, *n)
     assert "snapshots 4" in text
-    assert np.all(np.isfinite(snaps)) and np.abs(snaps[3]).max() > 0
+    for k, t in enumerate((0, 10, 20, 30)):
+        cur_t, _, _, _ = ofd.forward(c, vel_h, w, t + 1, rcv=rcv)
+        assert rel(snaps[k], cur_t[hw:-hw, hw:-hw, hw:-hw]) < 1e-5, t
+    # compute_energy: sum of the squares of the finite values of `current`
+    e_o = float((cur_o ** 2).sum())
+    energy = float(text.split(" energy ")[1].split()[0])            # reduced on the device
+    host_energy = float(text.split("host_energy ")[1].split()[0])   # from the host mirror
+    assert abs(energy - e_o) <= 2e-5 * e_o and abs(host_energy - e_o) <= 2e-5 * e_o
+    assert abs(energy - host_energy) <= 1e-12 * e_o
     # inject_source + step_forward by hand equals forward_propagate (the host-side
     # injection rounds the sum once more than the fused one)
     assert rel(f64(tmp_path, "cur_steps.f64"), f64(tmp_path, "cur.f64")) < 1e-6
@@ -99,6 +107,7 @@ def test_csr_sparse_ops_match_oracle(tmp_path, cuda):
     # FP32 matrix values on the device, FP64 vectors and accumulation
     assert rel(f64(tmp_path, "y.f64"), dense @ x) < 1e-6
     assert rel(f64(tmp_path, "yt.f64"), dense.T @ u) < 1e-6
+    assert rel(f64(tmp_path, "dd.f64"), (dense @ x)[: rows - 5]) < 1e-6
     # FD matrix vs the oracle's assembly of the same triplets, CG and PCG vs a dense solve
     frp = np.fromfile(tmp_path / "fd_rp.i64", np.int64)
     fci = np.fromfile(tmp_path / "fd_ci.i64", np.int64)
@@ -137,6 +146,8 @@ def test_rtm_engine_and_run_rtm(tmp_path, cuda):
     rcv.astype(np.int64).tofile(tmp_path / "rcv.i64")
     resid = rng.standard_normal((rcv.size, nt)) * 1e-3
     resid.tofile(tmp_path / "resid.f64")
+    rfield = rng.standard_normal(nh) * 1e-3
+    rfield.tofile(tmp_path / "resid_field.f64")
     text = run("rtm", tmp_path, nx=n[0], ny=n[1], nz=n[2], hw=hw, nt=nt)
     hs = tuple(slice(hw, hw + k) for k in n)
     fi = np.where(np.isfinite(f), f, 0)[hs].ravel()
@@ -169,6 +180,25 @@ def test_rtm_engine_and_run_rtm(tmp_path, cuda):
     c = ofd.cfg(n, hw=hw, order=8, sponge_width=6)
     _, _, tr_o, _ = ofd.forward(c, vel_h, w, nt, rcv=rcv)
     assert rel(f64(tmp_path, "rtm_traces.f64"), tr_o) < 1e-5
+    # the reference's flow (wave_equation.cpp:621-688): the forward field with
+    # itself after every step, then once with the residual field; finalize
+    assert "rtm_field status=OK steps=60 ok=1" in text
+    assert "rtm_short status=OK steps=0 ok=0" in text
+    img_o = np.zeros(int(np.prod(n)))
+    ill_o = np.zeros_like(img_o)
+    for t in range(nt):
+        cur_t, _, _, _ = ofd.forward(c, vel_h, w, t + 1)
+        ci = cur_t[hs].ravel()
+        img_o += ci * ci
+        ill_o += ci * ci
+    ci = cur_t[hs].ravel()
+    img_o += ci * rfield.reshape(nh)[hs].ravel()
+    ill_o += ci * ci
+    assert rel(f64(tmp_path, "rtmf_image.f64"), img_o) < 1e-5
+    assert rel(f64(tmp_path, "rtmf_illum.f64"), ill_o) < 1e-5
+    refl = f64(tmp_path, "rtmf_refl.f64")
+    on = ill_o > 1e-6 * ill_o.max()
+    assert rel(refl[on], (img_o / np.where(ill_o > 0, ill_o, 1))[on]) < 1e-4
 
 
 def test_fault_tolerance_on_device_matches_oracle(tmp_path, cuda):
@@ -209,3 +239,19 @@ def test_terrain_boundary_matches_reference(tmp_path, cuda):
     assert np.array_equal(f64(tmp_path, "free.f64"), ref["free"])
     assert np.array_equal(f64(tmp_path, "abs.f64"), ref["absorb"])
     assert f"seafloor_index {int(b[3 * N[1] + 4] / H[2])}" in text
+
+
+@pytest.mark.parametrize("sigma", [0.8, 1.5])
+def test_smooth_gradient_and_gaussian_filter(tmp_path, cuda, sigma):
+    """SparseOps::smooth_gradient against the oracle's direct 3-D sum
+    (sparse_ops.cpp:115-158) and gaussian_filter_3d against its closed form
+    (sparse_ops.cpp:212-234); a gradient shorter than the grid gives zeros."""
+    nx, ny, nz = 13, 11, 9
+    g = np.random.default_rng(5).standard_normal(nx * ny * nz)
+    g.tofile(tmp_path / "g.f64")
+    text = run("smooth", tmp_path, nx=nx, ny=ny, nz=nz, sigma=sigma)
+    assert rel(f64(tmp_path, "smooth.f64"), osp.smooth_gradient(g, nx, ny, nz, sigma)) < 1e-13
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    k = np.exp(-((ix - nx / 2.0) ** 2 + (iy - ny / 2.0) ** 2 + (iz - nz / 2.0) ** 2) / (2 * sigma ** 2)).ravel()
+    assert rel(f64(tmp_path, "gauss.f64"), k / k.sum()) < 1e-14
+    assert f"short_gradient size {nx * ny * nz} max 0" in text

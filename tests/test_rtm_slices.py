@@ -102,3 +102,65 @@ def test_gpu_slices_match_reference(HW, cuda):
         assert np.array_equal(s.forward_slice, f) and np.array_equal(s.adjoint_slice, a)
         assert np.array_equal(s.ic_slice, c)
         assert decode_slice_frame(encode_slice_frame(s))[0].index == s.index
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not REFLIB.exists(), reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("HW", [0, 4])
+def test_gpu_image_condition_and_finalize_match_reference(HW, cuda):
+    """stp_rtm_image_condition_f64 / stp_rtm_finalize_f64 against the
+    reference's RTMEngine::apply_image_condition, finalize_image and
+    reset_image (rtm_engine.cpp:24-57, 170-190) compiled from its own source:
+    three accumulations (non-finite inputs included), finalize, then a reset
+    and one more accumulation."""
+    import torch
+    from seismic_engine._lib import call, ptr, stream_handle
+    lib = C.CDLL(str(REFLIB))
+    lib.ref_rtm_create.restype = C.c_void_p
+    for f in ("ref_rtm_apply", "ref_rtm_finalize", "ref_rtm_reset", "ref_rtm_result", "ref_rtm_free"):
+        getattr(lib, f).restype = None
+    lib.ref_rtm_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64]
+    lib.ref_rtm_finalize.argtypes = lib.ref_rtm_reset.argtypes = lib.ref_rtm_free.argtypes = [C.c_void_p]
+    lib.ref_rtm_result.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    nyh, nzh = N[1] + 2 * HW, N[2] + 2 * HW
+    nh = (N[0] + 2 * HW) * nyh * nzh
+    ni = N[0] * N[1] * N[2]
+    rng = np.random.default_rng(7)
+    h = C.c_void_p(lib.ref_rtm_create(C.c_int64(N[0]), C.c_int64(N[1]), C.c_int64(N[2]), C.c_int(HW)))
+    dev = dict(dtype=torch.float64, device=cuda)
+    img, ill, refl = (torch.zeros(ni, **dev) for _ in range(3))
+
+    def ref_result():
+        out = [np.zeros(ni) for _ in range(3)]
+        lib.ref_rtm_result(h, *(o.ctypes.data_as(C.c_void_p) for o in out))
+        return out
+
+    def both(f, a, step):
+        lib.ref_rtm_apply(h, f.ctypes.data_as(C.c_void_p), a.ctypes.data_as(C.c_void_p), nyh, nzh, step)
+        ft, at = torch.as_tensor(f, **dev), torch.as_tensor(a, **dev)
+        call("stp_rtm_image_condition_f64", C.c_int64(N[0]), C.c_int64(N[1]), C.c_int64(N[2]), HW,
+             C.c_int64(nyh), C.c_int64(nzh), ptr(ft), ptr(at), ptr(img), ptr(ill), stream_handle())
+
+    for step in range(3):
+        f, a = rng.standard_normal(nh), rng.standard_normal(nh)
+        f[[3 + step, nh // 3]] = [np.nan, np.inf]
+        a[[11, nh - 2]] = [-np.inf, np.nan]
+        both(f, a, step)
+    # a cell with zero illumination: finalize must give 0 there
+    lib.ref_rtm_finalize(h)
+    call("stp_rtm_finalize_f64", C.c_int64(ni), ptr(img), ptr(ill), ptr(refl), stream_handle())
+    r_img, r_ill, r_refl = ref_result()
+    np.testing.assert_allclose(img.cpu().numpy(), r_img, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(ill.cpu().numpy(), r_ill, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(refl.cpu().numpy(), r_refl, rtol=1e-13, atol=1e-13)
+    # reset, then one accumulation of a field with itself (f = a)
+    lib.ref_rtm_reset(h)
+    img.zero_(); ill.zero_()
+    f = rng.standard_normal(nh)
+    both(f, f, 9)
+    lib.ref_rtm_finalize(h)
+    call("stp_rtm_finalize_f64", C.c_int64(ni), ptr(img), ptr(ill), ptr(refl), stream_handle())
+    r_img, r_ill, r_refl = ref_result()
+    lib.ref_rtm_free(h)
+    np.testing.assert_allclose(img.cpu().numpy(), r_img, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(refl.cpu().numpy(), r_refl, rtol=1e-13, atol=1e-13)
