@@ -217,6 +217,24 @@ int stp_csr_spmv_f64(const stp_csr* h, int32_t transpose, const double* x, doubl
     return STP_OK;
 }
 
+// Measurement: the gathers and index stream of one product through the
+// matrix the solver uses (the solver layout when built, else the given
+// order), without the value stream.
+int stp_csr_gather_probe(const stp_csr* h, int32_t transpose, int32_t fp64, const void* x, void* y, void* stream) {
+    if (!h || !x || !y) return fail(STP_ERR_INVALID, "stp_csr_gather_probe: null argument");
+    const bool lay = h->lay_ready;
+    const int n = transpose ? h->cols : h->rows;
+    const int* rp = transpose ? (lay ? h->ltrp : h->trp) : (lay ? h->lrp : h->rp);
+    const int* ci = transpose ? (lay ? h->ltci : h->tci) : (lay ? h->lci : h->ci);
+    const int W = transpose ? (lay ? h->lWT : h->WT) : (lay ? h->lW : h->W);
+    hipStream_t st = S(stream);
+    if (fp64)
+        STP_HIP(stp::gather_probe<double>(W, n, rp, ci, static_cast<const double*>(x), static_cast<double*>(y), st));
+    else
+        STP_HIP(stp::gather_probe<float>(W, n, rp, ci, static_cast<const float*>(x), static_cast<float*>(y), st));
+    return STP_OK;
+}
+
 // ------------------------------------------------------------------ square CG / Jacobi-PCG
 int stp_csr_set_values_f64(stp_csr* h, const double* val64, void* stream) {
     if (!h) return fail(STP_ERR_INVALID, "stp_csr_set_values_f64: null handle");
@@ -371,7 +389,8 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
     }
     const int nbq = std::max(stp::spmv_blocks(rows, W), 1);
     const int nbs = std::max(stp::spmv_blocks(cols, WT), 1);
-    const int nbc = std::max((cols + 255) / 256, 1);
+    const int nbc = std::max((cols + 255) / 256, 1);   // partials of the learned apply
+    const int npp = stp::cgls_blocks(cols);            // |p|^2 partials of the direction kernel
     const bool flex = precond == 2;
     float* dinv = nullptr;
     if (precond == 1) {
@@ -382,27 +401,27 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
     const size_t fw = sizeof(float) / 4, dw = sizeof(double) / 4;
 
     // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s
-    auto start = [&](int set_s0) -> int {
+    auto start = [&](int set_s0, int it0) -> int {
         STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3, nullptr,
                                      st));
         if (flex) {
             STP_HIP(stp::launch_zero_words(w.sp, fw * cols, st));
             int rc = stp::lpc_apply(precond_handle, w.s, w.z, nullptr, w.sp, w.part4, w.part5, st, perm);
             if (rc != STP_OK) return rc;
-            STP_HIP(stp::launch_cgls_init(w.sc, w.part4, nbc, w.part3, nbs, set_s0, st));
+            STP_HIP(stp::launch_cgls_init(w.sc, w.part4, nbc, w.part3, nbs, set_s0, it0, st));
         } else {
-            STP_HIP(stp::launch_cgls_init(w.sc, w.part2, nbs, w.part3, nbs, set_s0, st));
+            STP_HIP(stp::launch_cgls_init(w.sc, w.part2, nbs, w.part3, nbs, set_s0, it0, st));
         }
         STP_HIP(stp::launch_zero_words(w.p, fw * cols, st));
-        STP_HIP(stp::launch_zero_words(&w.sc->beta, dw, st));
-        STP_HIP(stp::launch_cgls_direction(w.sc, cols, w.z, w.p, w.part1, w.s, w.sp, st));
+        STP_HIP(stp::launch_cgls_beta_direction(w.sc, nullptr, 0, nullptr, 0, nullptr, 0, tol, nullptr, it0, 1, cols,
+                                                w.z, w.p, w.part1, w.s, w.sp, st));
         return STP_OK;
     };
 
     STP_HIP(stp::launch_zero_words(x, dw * cols, st));
     STP_HIP(stp::launch_zero_words(w.sc, sizeof(stp::SolverScalars) / 4, st));
     STP_HIP(hipMemcpyAsync(w.r, b, sizeof(double) * rows, hipMemcpyDeviceToDevice, st));
-    int rc = start(1);
+    int rc = start(1, 0);
     if (rc != STP_OK) return rc;
     stp::SolverScalars hs;
     std::memset(&hs, 0, sizeof hs);
@@ -412,18 +431,19 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
     for (;;) {
         for (; it < max_iter; ++it) {
             STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st));
-            STP_HIP(stp::launch_cgls_alpha(w.sc, w.part0, nbq, w.part1, nbc, lam, st));
-            STP_HIP(stp::launch_cgls_update(w.sc, cols, x, w.p, rows, w.r, w.q, st));
+            STP_HIP(stp::launch_cgls_alpha_update(w.sc, w.part0, nbq, w.part1, npp, lam, it, cols, x, w.p, rows, w.r,
+                                                  w.q, st));
             STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3,
                                          done, st));
             if (flex) {
                 rc = stp::lpc_apply(precond_handle, w.s, w.z, done, w.sp, w.part4, w.part5, st, perm);
                 if (rc != STP_OK) return rc;
-                STP_HIP(stp::launch_cgls_beta(w.sc, w.part4, nbc, w.part3, nbs, w.part5, nbc, tol, history, it, st));
+                STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part4, nbc, w.part3, nbs, w.part5, nbc, tol, history,
+                                                        it, 0, cols, w.z, w.p, w.part1, w.s, w.sp, st));
             } else {
-                STP_HIP(stp::launch_cgls_beta(w.sc, w.part2, nbs, w.part3, nbs, nullptr, 0, tol, history, it, st));
+                STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part2, nbs, w.part3, nbs, nullptr, 0, tol, history, it,
+                                                        0, cols, w.z, w.p, w.part1, w.s, nullptr, st));
             }
-            STP_HIP(stp::launch_cgls_direction(w.sc, cols, w.z, w.p, w.part1, w.s, flex ? w.sp : nullptr, st));
             if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
                 STP_HIP(hipMemcpyAsync(&hs, w.sc, sizeof hs, hipMemcpyDeviceToHost, st));
                 STP_HIP(hipStreamSynchronize(st));
@@ -444,7 +464,7 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
         true_rel = hs.s0 > 0 ? std::sqrt(ss) / hs.s0 : 0.0;
         if (hs.done != 1 || true_rel < tol || it >= max_iter || replacements >= 8) break;
         ++replacements;
-        rc = start(0);   // cgls_init clears the done flag before the direction kernel runs
+        rc = start(0, it);   // cgls_init clears the done flag before the direction kernel runs
         if (rc != STP_OK) return rc;
     }
     if (lay) STP_HIP(stp::permute<double>(cols, x, h->cperm, x_out, false, st));

@@ -9,6 +9,7 @@ struct SolverScalars {
     double alpha, beta, gamma, s0, rel;
     int done;    // 0 running, 1 converged, 2 breakdown
     int iters;
+    double gam[2];   // CGLS: gamma of iteration it in gam[it & 1]
 };
 
 // upper bound of the grid (and so of the per-block partial sums) of the SpMV
@@ -32,6 +33,9 @@ hipError_t spmv_f64v(int W, int rows, const int* rp, const int* ci, const double
                      const int* done, hipStream_t st);
 hipError_t residual_f64v(int W, int rows, const int* rp, const int* ci, const double* val, const double* x,
                          const double* b, double* r, hipStream_t st);
+// the SpMV without its value stream (measurement of the gather cost)
+template <typename XT>
+hipError_t gather_probe(int W, int rows, const int* rp, const int* ci, const XT* x, XT* y, hipStream_t st);
 
 hipError_t rays_count(int64_t nrays, const double* pts, const int64_t* n, double h, int* counts, hipStream_t st);
 hipError_t rays_fill(int64_t nrays, const double* pts, const int64_t* n, double h, const int* rp, int* cols,
@@ -40,18 +44,26 @@ hipError_t exclusive_scan_i32(const int* in, int* out, int n, hipStream_t st);
 hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const int* ci, const float* val, int* trp,
                          int* tci, float* tval, hipStream_t st);
 
-hipError_t launch_cgls_alpha(SolverScalars* sc, const double* pq, int nq, const double* pp, int np, double lam,
-                             hipStream_t st);
-hipError_t launch_cgls_beta(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
-                            const double* pflex, int nfl, double tol, double* history, int it, hipStream_t st);
-hipError_t launch_cgls_update(const SolverScalars* sc, int cols, double* x, const float* p, int rows, double* r,
-                              const float* q, hipStream_t st);
-hipError_t launch_cgls_direction(const SolverScalars* sc, int cols, const float* z, float* p, double* part_pp,
-                                 const float* s, float* s_prev, hipStream_t st);
+// CGLS iteration kernels.  cgls_blocks(n) bounds the grid of the
+// elementwise CGLS kernels (grid-stride), and so the count of |p|^2 partials.
+int cgls_blocks(int n);
+// alpha = gam[it&1] / (|q|^2 + lam |p|^2) (every block sums the partials),
+// x += alpha p, r -= alpha q
+hipError_t launch_cgls_alpha_update(SolverScalars* sc, const double* pq, int nq, const double* pp, int np,
+                                    double lam, int it, int cols, double* x, const float* p, int rows, double* r,
+                                    const float* q, hipStream_t st);
+// rel, convergence test, beta = (z.s or z.(s - s_prev)) / gam[it&1] (every
+// block), gam[(it+1)&1] = z.s, then p = z + beta p with |p|^2 partials and
+// s_prev <- s; with first = 1 (a start or restart) beta = 0 and no scalar is
+// read or written
+hipError_t launch_cgls_beta_direction(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
+                                      const double* pflex, int nfl, double tol, double* history, int it, int first,
+                                      int cols, const float* z, float* p, double* part_pp, const float* s,
+                                      float* s_prev, hipStream_t st);
 hipError_t launch_flex_dots(const int* done, int n, const float* z, const float* s, const float* s_prev,
                             double* part_zs, double* part_fl, hipStream_t st);
 hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss, int set_s0,
-                            hipStream_t st);
+                            int it, hipStream_t st);
 hipError_t launch_copy_f32(int n, const float* a, float* b, hipStream_t st);
 // zero n 32-bit words with a kernel (no memset node when a caller captures)
 hipError_t launch_zero_words(void* p, size_t nwords, hipStream_t st);

@@ -34,7 +34,7 @@ constexpr int MAX_BLOCKS = SPMV_MAX_BLOCKS;
 // instead of a chain of dependent load pairs.  Slots past the row end load
 // the row's first entry and contribute an exact zero, which leaves the sum
 // (and its order) unchanged.
-template <int W, typename AT, typename XT, typename VT = float>
+template <int W, typename AT, typename XT, typename VT = float, bool NOVAL = false>
 __device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const int* __restrict__ ci,
                                       const VT* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
     constexpr int U = (128 / W) < 1 ? 1 : (128 / W);
@@ -49,7 +49,8 @@ __device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const int* __r
             const int k = k0 + u * W;
             ok[u] = k < e;
             const int kk = ok[u] ? k : b;
-            v[u] = val[kk];
+            if constexpr (NOVAL) v[u] = (VT)1;
+            else v[u] = val[kk];
             c[u] = ci[kk];
         }
         AT xv[U];
@@ -189,6 +190,39 @@ hipError_t spmv_f64v(int W, int rows, const int* rp, const int* ci, const double
     }
 }
 
+// Gather probe: the SpMV kernel without its value stream (row sums of the
+// gathered x[ci[k]], same lanes per row, chunking and row order), i.e. the
+// index stream plus the gathers of one product.  Measurement only.
+template <int W, typename XT>
+__global__ void __launch_bounds__(256) gather_probe_kernel(int rows, const int* __restrict__ rp,
+                                                           const int* __restrict__ ci, const XT* __restrict__ x,
+                                                           XT* __restrict__ y) {
+    constexpr int RPB = 256 / W;
+    const int lane = threadIdx.x % W, sub = threadIdx.x / W;
+    for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
+        const int row = row0 + sub;
+        if (row < rows) {
+            const XT acc = row_dot<W, XT, XT, float, true>(rp, ci, nullptr, x, row, lane);
+            if (lane == 0) y[row] = acc;
+        }
+    }
+}
+
+template <typename XT>
+hipError_t gather_probe(int W, int rows, const int* rp, const int* ci, const XT* x, XT* y, hipStream_t st) {
+    if (rows == 0) return hipSuccess;
+    const int nb = spmv_blocks(rows, W);
+    switch (W) {
+        case 64: hipLaunchKernelGGL((gather_probe_kernel<64, XT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, x, y); break;
+        case 32: hipLaunchKernelGGL((gather_probe_kernel<32, XT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, x, y); break;
+        case 16: hipLaunchKernelGGL((gather_probe_kernel<16, XT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, x, y); break;
+        default: hipLaunchKernelGGL((gather_probe_kernel<8, XT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, x, y); break;
+    }
+    return hipGetLastError();
+}
+template hipError_t gather_probe<float>(int, int, const int*, const int*, const float*, float*, hipStream_t);
+template hipError_t gather_probe<double>(int, int, const int*, const int*, const double*, double*, hipStream_t);
+
 // s = A^T r - lam*x with FP64 r and x, FP32 output s, z = dinv*s, partials of s*z and s*s
 template <int W>
 __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const int* __restrict__ rp,
@@ -255,58 +289,84 @@ __device__ double block_sum(const double* part, int n) {
     return r;
 }
 
-// CGLS step length: alpha = gamma / (|q|^2 + lam |p|^2)
-__global__ void cgls_alpha_kernel(SolverScalars* sc, const double* pq, int nq, const double* pp, int np, double lam) {
-    if (sc->done) return;
+// CGLS, after q = A p (with its |q|^2 partials): alpha = gamma / (|q|^2 +
+// lam |p|^2), x += alpha p (cols), r -= alpha q (rows).  Every block sums the
+// same partials in the same order, so all blocks hold the same alpha and no
+// single-block scalar launch sits between the SpMV and the update; block 0
+// records alpha (and the breakdown flag).  The done flag is read once per
+// block (block 0 may set it while other blocks start).
+__global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* sc, const double* pq, int nq,
+                                                                const double* pp, int np, double lam, int it,
+                                                                int cols, double* __restrict__ x,
+                                                                const float* __restrict__ p, int rows,
+                                                                double* __restrict__ r, const float* __restrict__ q) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = sc->done;
+    __syncthreads();
+    if (s_done) return;
     const double qq = block_sum(pq, nq);
     const double ppv = block_sum(pp, np);
-    if (threadIdx.x == 0) {
-        const double del = qq + lam * ppv;
-        if (!(del > 0.0)) { sc->done = 2; sc->alpha = 0.0; return; }
-        sc->alpha = sc->gamma / del;
+    const double del = qq + lam * ppv;
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(del > 0.0)) {
+        if (first) { sc->done = 2; sc->alpha = 0.0; }
+        return;
+    }
+    const double a = sc->gam[it & 1] / del;
+    if (first) sc->alpha = a;
+    const int n = cols > rows ? cols : rows;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < cols) x[i] += a * (double)p[i];
+        if (i < rows) r[i] -= a * (double)q[i];
     }
 }
 
-// after s = A^T r - lam x: record |s|/s0, test convergence, beta = gamma'/gamma
-// (flexible variant: gamma' = z.s, beta = z.(s - s_prev)/gamma)
-__global__ void cgls_beta_kernel(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
-                                 const double* pflex, int nfl, double tol, double* history, int it) {
-    if (sc->done) return;
-    const double sz = block_sum(psz, nsz);
-    const double ss = block_sum(pss, nss);
-    const double fl = pflex ? block_sum(pflex, nfl) : 0.0;
-    if (threadIdx.x == 0) {
+// CGLS, after s = A^T r - lam x and z = M s (with the s.z, s.s and flexible
+// partials): rel = |s| / s0, the convergence test, beta = z.s / gamma (or
+// z.(s - s_prev) / gamma, flexible), then p = z + beta p with its |p|^2
+// partials and s_prev <- s.  gamma sits in two slots by iteration parity:
+// every block reads gam[it & 1] and block 0 writes gam[(it + 1) & 1], so no
+// block reads what another block of the launch writes.  FIRST (a start or a
+// restart): beta = 0, no scalar is read or written.
+template <bool FIRST>
+__global__ void __launch_bounds__(256) cgls_beta_direction_kernel(SolverScalars* sc, const double* psz, int nsz,
+                                                                  const double* pss, int nss, const double* pflex,
+                                                                  int nfl, double tol, double* history, int it,
+                                                                  int cols, const float* __restrict__ z,
+                                                                  float* __restrict__ p, double* __restrict__ part_pp,
+                                                                  const float* __restrict__ s,
+                                                                  float* __restrict__ s_prev) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = sc->done;
+    __syncthreads();
+    if (s_done) return;
+    double b = 0.0;
+    if constexpr (!FIRST) {
+        const double sz = block_sum(psz, nsz);
+        const double ss = block_sum(pss, nss);
+        const double fl = pflex ? block_sum(pflex, nfl) : 0.0;
         const double rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
-        sc->rel = rel;
-        sc->iters = it + 1;
-        if (history) history[it] = rel;
-        if (rel < tol) { sc->done = 1; return; }
-        sc->beta = (pflex ? fl : sz) / sc->gamma;
-        sc->gamma = sz;
+        const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+        if (first) {
+            sc->rel = rel;
+            sc->iters = it + 1;
+            if (history) history[it] = rel;
+        }
+        if (rel < tol) {
+            if (first) sc->done = 1;
+            return;
+        }
+        b = (pflex ? fl : sz) / sc->gam[it & 1];
+        if (first) {
+            sc->beta = b;
+            sc->gam[(it + 1) & 1] = sz;
+        }
     }
-}
-
-// x += alpha p (cols), r -= alpha q (rows)
-__global__ void cgls_update_kernel(const SolverScalars* sc, int cols, double* x, const float* p, int rows, double* r,
-                                   const float* q) {
-    if (sc->done) return;
-    const double a = sc->alpha;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cols) x[i] += a * (double)p[i];
-    if (i < rows) r[i] -= a * (double)q[i];
-}
-
-// p = z + beta p, partial |p|^2 ; optional s_prev <- s (flexible variant)
-__global__ void cgls_direction_kernel(const SolverScalars* sc, int cols, const float* z, float* p, double* part_pp,
-                                      const float* s, float* s_prev) {
-    if (sc->done) return;
-    const double b = sc->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
-    if (i < cols) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cols; i += gridDim.x * blockDim.x) {
         const float pv = (float)((double)z[i] + b * (double)p[i]);
         p[i] = pv;
-        v = (double)pv * (double)pv;
+        v += (double)pv * (double)pv;
         if (s_prev) s_prev[i] = s[i];
     }
     __shared__ double red[256];
@@ -347,11 +407,12 @@ __global__ void flex_dots_kernel(const int* done, int n, const float* z, const f
 // (restart from a recomputed residual: the done flag of the finished pass is
 // cleared here, so the direction kernel that follows runs)
 __global__ void cgls_init_kernel(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
-                                 int set_s0) {
+                                 int set_s0, int it) {
     const double sz = block_sum(psz, nsz);
     const double ss = block_sum(pss, nss);
     if (threadIdx.x == 0) {
         sc->gamma = sz;
+        sc->gam[it & 1] = sz;
         if (set_s0) { sc->s0 = sqrt(ss); sc->done = sc->s0 > 0.0 ? 0 : 1; sc->iters = 0; }
         else sc->done = 0;
         sc->rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
@@ -728,26 +789,27 @@ hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const i
 // ------------------------------------------------------------------ launch helpers
 static unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-hipError_t launch_cgls_alpha(SolverScalars* sc, const double* pq, int nq, const double* pp, int np, double lam,
-                             hipStream_t st) {
-    hipLaunchKernelGGL(cgls_alpha_kernel, dim3(1), dim3(256), 0, st, sc, pq, nq, pp, np, lam);
+int cgls_blocks(int n) {
+    int b = (n + 255) / 256;
+    return b > 256 ? 256 : (b < 1 ? 1 : b);
+}
+hipError_t launch_cgls_alpha_update(SolverScalars* sc, const double* pq, int nq, const double* pp, int np,
+                                    double lam, int it, int cols, double* x, const float* p, int rows, double* r,
+                                    const float* q, hipStream_t st) {
+    hipLaunchKernelGGL(cgls_alpha_update_kernel, dim3(cgls_blocks(cols > rows ? cols : rows)), dim3(256), 0, st, sc,
+                       pq, nq, pp, np, lam, it, cols, x, p, rows, r, q);
     return hipGetLastError();
 }
-hipError_t launch_cgls_beta(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
-                            const double* pflex, int nfl, double tol, double* history, int it, hipStream_t st) {
-    hipLaunchKernelGGL(cgls_beta_kernel, dim3(1), dim3(256), 0, st, sc, psz, nsz, pss, nss, pflex, nfl, tol, history,
-                       it);
-    return hipGetLastError();
-}
-hipError_t launch_cgls_update(const SolverScalars* sc, int cols, double* x, const float* p, int rows, double* r,
-                              const float* q, hipStream_t st) {
-    hipLaunchKernelGGL(cgls_update_kernel, dim3(nb256(cols > rows ? cols : rows)), dim3(256), 0, st, sc, cols, x, p,
-                       rows, r, q);
-    return hipGetLastError();
-}
-hipError_t launch_cgls_direction(const SolverScalars* sc, int cols, const float* z, float* p, double* part_pp,
-                                 const float* s, float* s_prev, hipStream_t st) {
-    hipLaunchKernelGGL(cgls_direction_kernel, dim3(nb256(cols)), dim3(256), 0, st, sc, cols, z, p, part_pp, s, s_prev);
+hipError_t launch_cgls_beta_direction(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
+                                      const double* pflex, int nfl, double tol, double* history, int it, int first,
+                                      int cols, const float* z, float* p, double* part_pp, const float* s,
+                                      float* s_prev, hipStream_t st) {
+    if (first)
+        hipLaunchKernelGGL(cgls_beta_direction_kernel<true>, dim3(cgls_blocks(cols)), dim3(256), 0, st, sc, psz, nsz,
+                           pss, nss, pflex, nfl, tol, history, it, cols, z, p, part_pp, s, s_prev);
+    else
+        hipLaunchKernelGGL(cgls_beta_direction_kernel<false>, dim3(cgls_blocks(cols)), dim3(256), 0, st, sc, psz,
+                           nsz, pss, nss, pflex, nfl, tol, history, it, cols, z, p, part_pp, s, s_prev);
     return hipGetLastError();
 }
 hipError_t launch_flex_dots(const int* done, int n, const float* z, const float* s, const float* s_prev,
@@ -756,8 +818,8 @@ hipError_t launch_flex_dots(const int* done, int n, const float* z, const float*
     return hipGetLastError();
 }
 hipError_t launch_cgls_init(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss, int set_s0,
-                            hipStream_t st) {
-    hipLaunchKernelGGL(cgls_init_kernel, dim3(1), dim3(256), 0, st, sc, psz, nsz, pss, nss, set_s0);
+                            int it, hipStream_t st) {
+    hipLaunchKernelGGL(cgls_init_kernel, dim3(1), dim3(256), 0, st, sc, psz, nsz, pss, nss, set_s0, it);
     return hipGetLastError();
 }
 hipError_t launch_zero_words(void* p, size_t nwords, hipStream_t st) {

@@ -77,6 +77,7 @@ SIGNATURES: dict[str, list] = {
     "stp_csr_set_grid": [_vp, _i64, _i64, _i64],
     "stp_csr_prepare_layout": [_vp, _vp],
     "stp_csr_set_values_f64": [_vp, _vp, _vp],
+    "stp_csr_gather_probe": [_vp, _i32, _i32, _vp, _vp, _vp],
     "stp_cg": [_vp, _vp, _i32, _f64, _i32, _vp, _vp, C.POINTER(SolveInfo), _vp],
     "stp_cgls": [_vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _vp, C.POINTER(SolveInfo), _vp],
     "stp_cgls_noalloc": [_vp, _vp, _f64, _f64, _i32, _i32, _vp, _vp, _vp, C.POINTER(SolveInfo), _vp],

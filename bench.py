@@ -116,24 +116,97 @@ def stream_3r1w_rate(dev, n: int, reps: int = 10) -> dict:
     return out
 
 
-def cpu_baseline(threads: int, vel_h, n: int, steps: int, sponge: str):
-    """FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x)
-    on the benchmark workload itself: the same n^3 model, order 8, sponge,
-    centre Ricker source and receivers, for the first `steps` time steps.
-    Returns the rate, the time and the final field and traces (the GPU's
-    first `steps` steps are compared with them)."""
+def cpu_baseline(threads: int, vel_h, n: int, steps: int, sponge: str, order: int = 8, rcv=None):
+    """The FP64 CPU restatement of the reference's forward propagation (its
+    4-pass step in C, OpenMP over x), built with the reference's release flags
+    (oracle liboracle_timing.so), on an n^3 with-halo model: `steps` time
+    steps from rest, centre Ricker source.  Returns (Gcell-updates/s, seconds)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
     omp_threads(threads)
     from oracle import fd as ofd
     from seismic_engine import synthetic
-    c = ofd.cfg((n, n, n), hw=4, order=8, sponge_width=20, sponge_damping=0.015, sponge=sponge)
-    w = synthetic.ricker(15.0, 1e-3, steps)
-    rcv = synthetic.line_receivers(n, n, n, 4, iy=n // 2, step=max(1, n // 256))
-    t0 = time.perf_counter()
-    cur, _, tr, res = ofd.forward(c, vel_h, w, steps, rcv=rcv)
-    dt = time.perf_counter() - t0
-    assert res.completed_successfully == 1
-    return float(n) ** 3 * steps / dt / 1e9, dt, cur, tr
+    ofd.select_build("timing")
+    try:
+        hw = 4 if order == 8 else 1
+        c = ofd.cfg((n, n, n), hw=hw, order=order, sponge_width=20, sponge_damping=0.015, sponge=sponge)
+        w = synthetic.ricker(15.0, 1e-3, steps)
+        t0 = time.perf_counter()
+        _, _, _, res = ofd.forward(c, vel_h, w, steps, rcv=rcv)
+        dt = time.perf_counter() - t0
+        assert res.completed_successfully == 1
+    finally:
+        ofd.select_build("check")
+    return float(n) ** 3 * steps / dt / 1e9, dt
+
+
+def cpu_fd_reference_order(threads: int, steps1: int = 50):
+    """The reference's own stencil (2nd order, 7 points, halo 1) on its
+    two-layer 128^3 case: 1500 / 2500 m/s split at iz = 64, sponge 20 / 0.015,
+    Ricker 15 Hz at the centre, receivers at every surface cell, 500 steps in
+    full on `threads` threads, and the first steps1 steps on one thread."""
+    from seismic_engine import synthetic
+    n = 128
+    vel_h = synthetic.pad_halo(synthetic.two_layer_velocity(n, n, n, split_iz=64), 1)
+    rcv = synthetic.surface_receivers(n, n, n, 1)
+    g, sec = cpu_baseline(threads, vel_h, n, 500, "reference", order=2, rcv=rcv)
+    g1, sec1 = cpu_baseline(1, vel_h, n, steps1, "reference", order=2, rcv=rcv)
+    omp_threads(threads)
+    return {"value": round(g, 5), "unit": "Gcell-updates/s", "cores": threads, "seconds": round(sec, 3),
+            "single_core_value": round(g1, 5),
+            "sample": f"two-layer 128^3, order 2 (the reference's 7-point stencil, halo 1), reference sponge, "
+                      f"500 steps in full on {threads} threads ({sec:.1f} s); one thread: first {steps1} steps "
+                      f"({sec1:.1f} s)"}
+
+
+def cpu_cg_baseline(dev, threads: int, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: float = 1e-6):
+    """The tomography solve on the CPU: the 200k-ray 64^3 system of the GPU
+    benchmark (FP32 path lengths as FP64, int64 indices), the reference's
+    2-accumulator SpMV (OpenMP over rows) and sequential scatter SpMV^T, and
+    Jacobi CGLS to 1e-6 (FP64 throughout), timed with the reference's release
+    flags.  SpMV bytes per call: 16 nnz + 8 (rows + 1) + 8 rows + 8 cols (x
+    counted once)."""
+    import numpy as np
+    import torch
+    from oracle import fd as ofd
+    from oracle import sparse as osp
+    from seismic_engine import synthetic
+    from seismic_engine.sparse import DeviceCSR
+    pts = synthetic.ray_endpoints(nrays, (n, n, n), h=1.0, seed=2026)
+    A = DeviceCSR.from_rays(pts, (n, n, n), 1.0)
+    s_true = torch.as_tensor(1.0 / synthetic.smooth_random_velocity(n, n, n, seed=2027, sigma=20.0 * n / 512 * 4),
+                             dtype=torch.float64, device=dev).reshape(-1)
+    b = A.matvec(s_true).cpu().numpy()
+    rp, ci, v = (t.cpu().numpy() for t in A.export())
+    rp, ci, v = rp.astype(np.int64), ci.astype(np.int64), v.astype(np.float64)
+    rows, cols, nnz = A.rows, A.cols, A.nnz
+    del A
+    x = np.random.default_rng(0).standard_normal(cols)
+    nbytes = 16 * nnz + 8 * (rows + 1) + 8 * rows + 8 * cols
+    out = {"rows": rows, "cols": cols, "nnz": nnz, "lambda": lam, "tol": tol}
+    ofd.select_build("timing")
+    try:
+        def timed(fn, reps):
+            fn()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            return (time.perf_counter() - t0) / reps
+        for k in (threads, 1):
+            omp_threads(k)
+            sec = timed(lambda: osp.spmv(rp, ci, v, x), 10 if k > 1 else 3)
+            out[f"spmv_{k}_threads"] = {"ms": round(sec * 1e3, 3), "gb_s": round(nbytes / sec / 1e9, 2)}
+        omp_threads(threads)
+        r = np.random.default_rng(1).standard_normal(rows)
+        sec = timed(lambda: osp.spmv_t(rp, ci, v, r, cols), 3)
+        out["spmv_t_sequential"] = {"ms": round(sec * 1e3, 3), "gb_s": round(nbytes / sec / 1e9, 2)}
+        t0 = time.perf_counter()
+        _, it, rel, _ = osp.cgls(rp, ci, v, b, cols, lam, tol, 20000, precond=1)
+        sec = time.perf_counter() - t0
+        out["jacobi_cgls"] = {"seconds": round(sec, 3), "iterations": int(it), "rel_residual": rel,
+                              "threads": threads}
+    finally:
+        ofd.select_build("check")
+    return out
 
 
 def accuracy_check(dev, threads: int, vel_int, sponge: str, case: str, steps: int = 500):
@@ -242,6 +315,30 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
     spmv_bytes = 8 * A.nnz + 4 * (A.rows + 1) + 4 * A.rows + 4 * A.cols
     out["spmv"] = {"us": round(us, 2), "gb_s": round(spmv_bytes / (us * 1e-6) / 1e9, 1),
                    "frac_hbm": round(spmv_bytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    # gather cost of the two CGLS products in the solver layout: the same
+    # kernels without the value stream (index stream + gathers), i.e. the
+    # ceiling of each product's gather pattern
+    from seismic_engine._lib import call, ptr, stream_handle
+    gp = {}
+    for name, transpose, fp64 in (("A_p_fp32", 0, 0), ("At_r_fp64", 1, 1)):
+        xin = torch.rand(A.rows if transpose else A.cols, dtype=torch.float64 if fp64 else torch.float32, device=dev)
+        yo = torch.empty(A.cols if transpose else A.rows, dtype=xin.dtype, device=dev)
+
+        def run():
+            call("stp_csr_gather_probe", A._h, transpose, fp64, ptr(xin), ptr(yo), stream_handle())
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        gus = e0.elapsed_time(e1) * 1e3 / 20
+        idx_bytes = 4 * A.nnz + 4 * ((A.cols if transpose else A.rows) + 1)
+        gp[name] = {"us": round(gus, 2), "gathers_per_s": round(A.nnz / (gus * 1e-6) / 1e9, 2),
+                    "unit": "G gathers/s", "index_stream_gb_s": round(idx_bytes / (gus * 1e-6) / 1e9, 1)}
+    out["gather_probe"] = gp
     from seismic_engine.learned import DEFAULT_WEIGHTS, LearnedPreconditioner, train_recipe
     if "learned" not in kinds:
         net = None
@@ -670,12 +767,6 @@ def main() -> int:
     assert torch.isfinite(tr).all()
     tr_max = float(tr.abs().max())
     field_max = float(fd.cur.abs().max())
-    spot = None
-    if not args.no_cpu_baseline:
-        # the first cpu-steps steps again, for the 512^3 spot check against the FP64 oracle
-        fd.reset()
-        spot = (fd.propagate(args.cpu_steps, w[: args.cpu_steps]).cpu().numpy(), fd.cur[0].cpu().numpy())
-
     cells = float(n) ** 3
     value = cells * nt * args.steps / wall / 1e9
     launch_s = sum(prop_ms) / 1e3 / (args.steps * nt)
@@ -745,27 +836,27 @@ def main() -> int:
         else:
             vel_h = vel_keep
         steps = args.cpu_steps
-        log(f"cpu baseline: FP64 oracle on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
-        gcs, sec, cur_o, tr_o = cpu_baseline(args.cpu_threads, vel_h, n, steps, args.sponge)
+        log(f"cpu baseline: FP64 restatement on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
+        gcs, sec = cpu_baseline(args.cpu_threads, vel_h, n, steps, args.sponge, rcv=rcv)
         steps1 = 2
         log(f"cpu baseline: the same on 1 core, first {steps1} steps")
-        gcs1, sec1, _, _ = cpu_baseline(1, vel_h, n, steps1, args.sponge)
+        gcs1, sec1 = cpu_baseline(1, vel_h, n, steps1, args.sponge, rcv=rcv)
         omp_threads(args.cpu_threads)
         del vel_h
-        tr_g, cur_g = spot
-        result["spot_check_512"] = {
-            "steps": steps, "field_rel_l2_vs_fp64": float(np.linalg.norm(cur_g - cur_o) / np.linalg.norm(cur_o)),
-            "traces_rel_l2_vs_fp64": float(np.linalg.norm(tr_g[0] - tr_o) / max(np.linalg.norm(tr_o), 1e-300))}
-        del cur_o, tr_o, cur_g, tr_g
         result["cpu_baseline"] = {
             "value": round(gcs, 5), "unit": "Gcell-updates/s", "cores": args.cpu_threads, "kind": "port",
-            "sample": f"FP64 oracle (the reference's 4-pass step restated in C, OpenMP over x; gcc -O3 "
-                      f"-march=x86-64-v3 without -ffast-math, since the same build is the parity checker) on the "
-                      f"same {n}^3 smooth random model, order 8, sponge 20, {len(rcv)} receivers: first {steps} of "
-                      f"{nt} steps, {sec:.1f} s",
+            "sample": f"FP64 C restatement of the reference's 4-pass step (OpenMP over x), built with the "
+                      f"reference's release flags (-O3 -ffast-math -mavx2 -mfma; x86-64-v3 for -march=native), on "
+                      f"the same {n}^3 smooth random model, order 8, sponge 20, {len(rcv)} receivers: first {steps} "
+                      f"of {nt} steps, {sec:.1f} s",
             "single_core_value": round(gcs1, 5),
             "single_core_sample": f"the same on 1 thread, first {steps1} steps, {sec1:.1f} s",
             "host": host_cpu()}
+        log("cpu baseline: the reference's 7-point stencil on the two-layer 128^3 case, 500 steps")
+        result["cpu_baseline"]["fd_order2_c1"] = cpu_fd_reference_order(args.cpu_threads)
+        if not args.no_cg:
+            log("cpu baseline: SpMV, SpMV^T and Jacobi CGLS on the 200k-ray 64^3 system")
+            result["cpu_baseline"]["cg"] = cpu_cg_baseline(dev, args.cpu_threads)
         log("accuracy: 128^3, 500 steps, FP32 GPU vs FP64 oracle")
         result["accuracy"] = [
             accuracy_check(dev, args.cpu_threads, synthetic.two_layer_velocity(128, 128, 128, split_iz=64),

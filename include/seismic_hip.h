@@ -207,6 +207,15 @@ int stp_csr_set_grid(stp_csr* A, int64_t nx, int64_t ny, int64_t nz);
 /* Build that layout now (no-op without a grid or when built).  Synchronises. */
 int stp_csr_prepare_layout(stp_csr* A, void* stream);
 
+/* Measurement, no counterpart in the reference: the gathers and the index
+ * stream of one product y = A x (transpose = 0) or A^T x (1) with FP32
+ * (fp64 = 0) or FP64 x, through the matrix CGLS uses (the solver layout when
+ * built, in its row and column order) and the same kernel shape, without the
+ * matrix values: y holds row sums of the gathered x.  The gather cost the
+ * sparse products are measured against. */
+int stp_csr_gather_probe(const stp_csr* A, int32_t transpose, int32_t fp64, const void* x_dev, void* y_dev,
+                         void* stream);
+
 /* Attach the FP64 values of A (device, nnz entries in the order of the
  * values given to stp_csr_create) so that stp_cg runs in the reference's
  * all-FP64 arithmetic (CSRMatrix keeps FP64 values: csr_matrix.h:68).  The

@@ -1,6 +1,8 @@
 """Build recipe for the CPU oracle (test infrastructure only).
 
   liboracle.so  — the FP64 C restatement in oracle/*.c (gcc, OpenMP)
+  liboracle_timing.so — the same sources with the reference's release flags,
+                  for the CPU baseline timings only
   _ref/         — the reference's own C++ sources that compile as they are,
                   built from /root/reference when it is present (never copied)
 
@@ -21,6 +23,13 @@ LIB = HERE / "liboracle.so"
 REF_OUT = HERE / "_ref"
 
 CFLAGS = ["-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-std=c11", "-Wall", "-Wno-unknown-pragmas"]
+# timing-only copy for the CPU baseline: the reference's release flags
+# (CMakeLists.txt:25, 31, 33: -O3 -march=native -ffast-math -mavx2 -mfma) with
+# -march=x86-64-v3 for native, since it is built here and timed on another
+# host; -ffast-math drops the finite checks, so it is never the checker
+TIMING_LIB = HERE / "liboracle_timing.so"
+TIMING_FLAGS = ["-O3", "-march=x86-64-v3", "-mavx2", "-mfma", "-ffast-math", "-fopenmp", "-fPIC", "-shared",
+                "-std=c11", "-Wno-unknown-pragmas"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -31,9 +40,10 @@ def _run(cmd: list[str]) -> None:
 
 def build_oracle() -> Path:
     srcs = sorted(str(p) for p in HERE.glob("*.c"))
-    tmp = LIB.with_suffix(".so.tmp")
-    _run(["gcc", *CFLAGS, *srcs, "-lm", "-o", str(tmp)])
-    os.replace(tmp, LIB)
+    for out, flags in ((LIB, CFLAGS), (TIMING_LIB, TIMING_FLAGS)):
+        tmp = out.with_suffix(".so.tmp")
+        _run(["gcc", *flags, *srcs, "-lm", "-o", str(tmp)])
+        os.replace(tmp, out)
     return LIB
 
 

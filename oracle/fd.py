@@ -7,7 +7,18 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-_lib = None
+_libs: dict = {}
+_build = "check"
+
+
+def select_build(kind: str) -> None:
+    """"check" (default): liboracle.so, the parity checker; "timing":
+    liboracle_timing.so, the same sources built with the reference's release
+    flags (-ffast-math), for the CPU baseline timings only."""
+    global _build
+    if kind not in ("check", "timing"):
+        raise ValueError(kind)
+    _build = kind
 
 
 class OrcCfg(C.Structure):
@@ -26,18 +37,18 @@ class OrcResult(C.Structure):
 
 
 def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        p = HERE / "liboracle.so"
+    if _build not in _libs:
+        p = HERE / ("liboracle.so" if _build == "check" else "liboracle_timing.so")
         if not p.exists():
             from . import build as _b
             _b.build_oracle()
-        _lib = C.CDLL(str(p))
-        _lib.orc_fd_stability_limit.restype = C.c_double
-        _lib.orc_fd_sponge_factor.restype = C.c_double
-        _lib.orc_count_nan_inf.restype = C.c_int64
-        _lib.orc_repair_field_nan.restype = C.c_int64
-    return _lib
+        h = C.CDLL(str(p))
+        h.orc_fd_stability_limit.restype = C.c_double
+        h.orc_fd_sponge_factor.restype = C.c_double
+        h.orc_count_nan_inf.restype = C.c_int64
+        h.orc_repair_field_nan.restype = C.c_int64
+        _libs[_build] = h
+    return _libs[_build]
 
 
 SPONGE_MODES = {"reference": 0, "cerjan": 1}
