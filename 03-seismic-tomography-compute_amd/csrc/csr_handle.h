@@ -16,10 +16,10 @@ struct stp_csr {
     // workspace
     double *r = nullptr, *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
     double *part4 = nullptr, *part5 = nullptr;
-    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr;
+    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr, *r32 = nullptr;
     double *dp = nullptr, *dap = nullptr, *dz = nullptr, *dr = nullptr, *ddinv = nullptr;
     stp::SolverScalars* sc = nullptr;
-    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0;
+    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0, c_r32 = 0;
     // solver layout (solver_layout.hip): columns are the cells of a
     // gn[0]*gn[1]*gn[2] grid when gn[0] > 0; the renumbered copy is built on
     // the first CGLS solve or by stp_csr_prepare_layout
@@ -30,5 +30,13 @@ struct stp_csr {
     float *lval = nullptr, *ltval = nullptr;
     double *lx = nullptr, *lb = nullptr;
     int lW = 16, lWT = 8;   // lanes per row in the layout
+    // staged copy of the layout A^T: its rows cut into tiles of st_rt rows;
+    // per tile the sorted distinct columns its entries read
+    // (st_uniq[st_tu[t] .. st_tu[t + 1])) and per entry the position of its
+    // column in that list (st_l16); st_rt = 0 when no tile size fits the
+    // staging buffer
+    int st_rt = 0, st_ntiles = 0, st_maxu = 0;
+    int *st_tu = nullptr, *st_uniq = nullptr;
+    uint16_t* st_l16 = nullptr;
     size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;
 };

@@ -16,6 +16,9 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <vector>
+
 #include "capi_common.h"
 #include "csr_handle.h"
 #include "seismic_hip.h"
@@ -107,6 +110,47 @@ __global__ void fill_kernel(int rows, const int* __restrict__ rsrc, const int* _
     }
 }
 
+// staged A^T: key (tile, column) of every entry, one wave per row
+__global__ void stage_key_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci, int rt,
+                                 uint64_t* __restrict__ key, int* __restrict__ idx) {
+    const int k = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= rows) return;
+    const uint64_t t = (uint64_t)(k / rt) << 32;
+    for (int j = rp[k] + lane; j < rp[k + 1]; j += 64) {
+        key[j] = t | (uint32_t)ci[j];
+        idx[j] = j;
+    }
+}
+
+// after the sort: flag the first entry of each distinct key (flag[nnz] = 0)
+// and count the distinct keys of each tile
+__global__ void stage_flag_kernel(int nnz, const uint64_t* __restrict__ key, int* __restrict__ flag,
+                                  int* __restrict__ cnt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nnz) return;
+    if (j == nnz) {
+        flag[j] = 0;
+        return;
+    }
+    const int f = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
+    flag[j] = f;
+    if (f) atomicAdd(&cnt[key[j] >> 32], 1);
+}
+
+// uidx = exclusive scan of flag, tu = exclusive scan of the tile counts: the
+// entry's distinct key is number uidx + flag - 1, its tile's list starts at tu
+__global__ void stage_fill_kernel(int nnz, const uint64_t* __restrict__ key, const int* __restrict__ flag,
+                                  const int* __restrict__ uidx, const int* __restrict__ idx,
+                                  const int* __restrict__ tu, int* __restrict__ uniq, uint16_t* __restrict__ l16) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nnz) return;
+    const int t = (int)(key[j] >> 32);
+    const int u = uidx[j] + flag[j] - 1;
+    if (flag[j]) uniq[u] = (int)(uint32_t)key[j];
+    l16[idx[j]] = (uint16_t)(u - tu[t]);
+}
+
 template <typename K>
 hipError_t sort_pairs(K* k0, K* k1, int* v0, int* v1, int n, int end_bit, hipStream_t st) {
     size_t tb = 0;
@@ -122,15 +166,78 @@ hipError_t sort_pairs(K* k0, K* k1, int* v0, int* v1, int n, int end_bit, hipStr
     return e;
 }
 
+// Staged copy of the layout A^T (n rows, entries reading a vector of length
+// g): the largest tile height in {256, 128, 64, 32} rows whose tiles each
+// read at most STAGE_CAP distinct entries of that vector.  None fitting
+// leaves st_rt = 0 (the CGLS iteration then gathers from global memory).
+hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
+    const int n = h->cols;
+    const int nnz = (int)h->nnz;
+    if (n == 0 || nnz == 0) return hipSuccess;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    int *i0 = nullptr, *i1 = nullptr, *flag = nullptr, *uidx = nullptr, *cnt = nullptr, *tu = nullptr;
+    hipError_t e = hipSuccess;
+    do {
+        if ((e = hipMalloc(&k0, sizeof(uint64_t) * nnz)) != hipSuccess) break;
+        if ((e = hipMalloc(&k1, sizeof(uint64_t) * nnz)) != hipSuccess) break;
+        if ((e = hipMalloc(&i0, sizeof(int) * nnz)) != hipSuccess) break;
+        if ((e = hipMalloc(&i1, sizeof(int) * nnz)) != hipSuccess) break;
+        if ((e = hipMalloc(&flag, sizeof(int) * ((size_t)nnz + 1))) != hipSuccess) break;
+        if ((e = hipMalloc(&uidx, sizeof(int) * ((size_t)nnz + 1))) != hipSuccess) break;
+        for (int rt = 256; rt >= 32; rt /= 2) {
+            const int nt = (n + rt - 1) / rt;
+            hipFree(cnt);
+            hipFree(tu);
+            cnt = tu = nullptr;
+            if ((e = hipMalloc(&cnt, sizeof(int) * ((size_t)nt + 1))) != hipSuccess) break;
+            if ((e = hipMalloc(&tu, sizeof(int) * ((size_t)nt + 1))) != hipSuccess) break;
+            if ((e = hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)nt + 1), st)) != hipSuccess) break;
+            hipLaunchKernelGGL(stage_key_kernel, dim3(blocks((int64_t)n * 64)), dim3(256), 0, st, n, h->ltrp, h->ltci,
+                               rt, k0, i0);
+            if ((e = hipGetLastError()) != hipSuccess) break;
+            if ((e = sort_pairs(k0, k1, i0, i1, nnz, 32 + bits_for((int64_t)nt), st)) != hipSuccess) break;
+            hipLaunchKernelGGL(stage_flag_kernel, dim3(blocks((int64_t)nnz + 1)), dim3(256), 0, st, nnz, k1, flag,
+                               cnt);
+            if ((e = hipGetLastError()) != hipSuccess) break;
+            if ((e = exclusive_scan_i32(cnt, tu, nt + 1, st)) != hipSuccess) break;
+            std::vector<int> htu((size_t)nt + 1);
+            if ((e = hipMemcpy(htu.data(), tu, sizeof(int) * ((size_t)nt + 1), hipMemcpyDeviceToHost)) != hipSuccess)
+                break;
+            int maxu = 0;
+            for (int t = 0; t < nt; ++t) maxu = std::max(maxu, htu[(size_t)t + 1] - htu[(size_t)t]);
+            if (maxu > STAGE_CAP) continue;
+            if ((e = exclusive_scan_i32(flag, uidx, nnz + 1, st)) != hipSuccess) break;
+            if ((e = hipMalloc(&h->st_uniq, sizeof(int) * (size_t)std::max(htu[(size_t)nt], 1))) != hipSuccess) break;
+            if ((e = hipMalloc(&h->st_l16, sizeof(uint16_t) * (size_t)nnz)) != hipSuccess) break;
+            hipLaunchKernelGGL(stage_fill_kernel, dim3(blocks(nnz)), dim3(256), 0, st, nnz, k1, flag, uidx, i1, tu,
+                               h->st_uniq, h->st_l16);
+            if ((e = hipGetLastError()) != hipSuccess) break;
+            h->st_tu = tu;
+            tu = nullptr;
+            h->st_rt = rt;
+            h->st_ntiles = nt;
+            h->st_maxu = maxu;
+            break;
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    } while (0);
+    hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(flag); hipFree(uidx); hipFree(cnt); hipFree(tu);
+    return e;
+}
+
 }  // namespace
 
 void free_solver_layout(stp_csr* h) {
-    void* ps[] = {h->cperm, h->rsrc, h->lrp, h->lci, h->lval, h->ltrp, h->ltci, h->ltval, h->lx, h->lb};
+    void* ps[] = {h->cperm, h->rsrc, h->lrp, h->lci, h->lval, h->ltrp, h->ltci, h->ltval, h->lx, h->lb,
+                  h->st_tu, h->st_uniq, h->st_l16};
     for (void* p : ps)
         if (p) hipFree(p);
     h->cperm = h->rsrc = h->lrp = h->lci = h->ltrp = h->ltci = nullptr;
     h->lval = h->ltval = nullptr;
     h->lx = h->lb = nullptr;
+    h->st_tu = h->st_uniq = nullptr;
+    h->st_l16 = nullptr;
+    h->st_rt = h->st_ntiles = h->st_maxu = 0;
     h->lay_ready = false;
 }
 
@@ -191,6 +298,7 @@ int build_solver_layout(stp_csr* h, hipStream_t st) {
             break;
         h->lW = spmv_lanes_layout(rows ? (double)nnz / rows : 0.0, 0);
         h->lWT = spmv_lanes_layout(cols ? (double)nnz / cols : 0.0, 1);
+        if ((e = build_staged_transpose(h, st)) != hipSuccess) break;
         e = hipStreamSynchronize(st);
     } while (0);
     hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(len);

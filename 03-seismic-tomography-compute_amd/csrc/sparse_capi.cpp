@@ -27,6 +27,11 @@ using stp::S;
 
 namespace {
 constexpr int CHECK = 8;   // iterations between host reads of the done flag
+// CGLS: the smallest tolerance solved with r gathered in FP32 for A^T r, and
+// the iterations without a 10 % drop of the residual after which such a pass
+// counts as stalled
+constexpr double R32_MIN_TOL = 1e-6;
+constexpr int STALL = 512;
 
 template <typename T>
 hipError_t grow(T** p, size_t* cap, size_t n) {
@@ -43,7 +48,7 @@ hipError_t grow(T** p, size_t* cap, size_t n) {
 
 static void csr_free(stp_csr* h) {
     void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->val64, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
-                  h->p, h->q, h->s, h->z, h->sp, h->dinv, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
+                  h->p, h->q, h->s, h->z, h->sp, h->dinv, h->r32, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
     for (void* p : ps)
         if (p) hipFree(p);
     stp::free_solver_layout(h);
@@ -350,7 +355,7 @@ namespace {
 // and the device-resident scalars.
 struct CglsWork {
     double* r = nullptr;
-    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr;
+    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr, *r32 = nullptr;
     double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *part4 = nullptr,
            *part5 = nullptr;
     stp::SolverScalars* sc = nullptr;
@@ -428,31 +433,64 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
     int it = 0, replacements = 0;
     double true_rel = 0.0;
     double ps[stp::SPMV_MAX_BLOCKS];
+    // For tol >= R32_MIN_TOL the iteration's A^T r gathers r rounded to FP32
+    // (r itself stays FP64, the sum is FP64): in the solver layout from the
+    // staged A^T, whose tiles read their distinct rays from r once, into LDS.
+    // That rounding bounds the attainable residual near 1e-7 of |A^T b| on
+    // systems whose residual r stays large, so a pass that stalls (no 10 %
+    // gain in STALL iterations) or ends above tol in the FP64 check restarts
+    // from the recomputed residual with r gathered in FP64.  Odd iterations
+    // sweep the rows of both products last to first, so each product starts
+    // on the part of its matrix that the previous sweep left in the device
+    // cache.
+    bool use32 = tol >= R32_MIN_TOL;
+    const bool staged = lay && h->st_rt > 0;
     for (;;) {
+        const int nbs_it = use32 && staged ? stp::staged_blocks(h->st_ntiles) : nbs;   // partials of A^T r
+        double best = HUGE_VAL;
+        int improved_at = it;
+        bool stalled = false;
         for (; it < max_iter; ++it) {
-            STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st));
+            const int rev = it & 1;
+            STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st, rev));
             STP_HIP(stp::launch_cgls_alpha_update(w.sc, w.part0, nbq, w.part1, npp, lam, it, cols, x, w.p, rows, w.r,
-                                                  w.q, st));
-            STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3,
-                                         done, st));
+                                                  w.q, use32 ? w.r32 : nullptr, st));
+            if (use32 && staged)
+                STP_HIP(stp::normal_residual_staged(WT, cols, h->st_rt, h->st_ntiles, h->st_maxu, TRP, h->st_l16,
+                                                    TVAL, h->st_tu, h->st_uniq, w.r32, x, lam, dinv, w.s, w.z,
+                                                    w.part2, w.part3, done, rev, st));
+            else if (use32)
+                STP_HIP(stp::normal_residual_r32(WT, cols, TRP, TCI, TVAL, w.r32, x, lam, dinv, w.s, w.z, w.part2,
+                                                 w.part3, done, rev, st));
+            else
+                STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3,
+                                             done, st, rev));
             if (flex) {
                 rc = stp::lpc_apply(precond_handle, w.s, w.z, done, w.sp, w.part4, w.part5, st, perm);
                 if (rc != STP_OK) return rc;
-                STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part4, nbc, w.part3, nbs, w.part5, nbc, tol, history,
-                                                        it, 0, cols, w.z, w.p, w.part1, w.s, w.sp, st));
+                STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part4, nbc, w.part3, nbs_it, w.part5, nbc, tol,
+                                                        history, it, 0, cols, w.z, w.p, w.part1, w.s, w.sp, st));
             } else {
-                STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part2, nbs, w.part3, nbs, nullptr, 0, tol, history, it,
-                                                        0, cols, w.z, w.p, w.part1, w.s, nullptr, st));
+                STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part2, nbs_it, w.part3, nbs_it, nullptr, 0, tol,
+                                                        history, it, 0, cols, w.z, w.p, w.part1, w.s, nullptr, st));
             }
             if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
                 STP_HIP(hipMemcpyAsync(&hs, w.sc, sizeof hs, hipMemcpyDeviceToHost, st));
                 STP_HIP(hipStreamSynchronize(st));
                 if (hs.done) break;
+                if (hs.rel < 0.9 * best) {
+                    best = hs.rel;
+                    improved_at = it;
+                } else if (use32 && it - improved_at >= STALL) {
+                    stalled = true;
+                    ++it;
+                    break;
+                }
             }
         }
         STP_HIP(hipMemcpyAsync(&hs, w.sc, sizeof hs, hipMemcpyDeviceToHost, st));
         STP_HIP(hipStreamSynchronize(st));
-        it = hs.iters;
+        if (!stalled) it = hs.iters;
         // true residual in FP64
         STP_HIP(stp::residual(W, rows, RP, CI, VAL, x, b, w.r, st));
         STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3, nullptr,
@@ -462,7 +500,9 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
         double ss = 0.0;
         for (int k = 0; k < nbs; ++k) ss += ps[k];
         true_rel = hs.s0 > 0 ? std::sqrt(ss) / hs.s0 : 0.0;
-        if (hs.done != 1 || true_rel < tol || it >= max_iter || replacements >= 8) break;
+        if (hs.done == 1 && true_rel < tol) break;
+        if (it >= max_iter || replacements >= 8 || (hs.done != 1 && !stalled)) break;
+        if (use32) use32 = false;   // the rest in FP64 gathers of r
         ++replacements;
         rc = start(0, it);   // cgls_init clears the done flag before the direction kernel runs
         if (rc != STP_OK) return rc;
@@ -499,7 +539,8 @@ constexpr size_t NA_OFF_S = NA_OFF_P + na_align(4 * NA_MAX_COLS);
 constexpr size_t NA_OFF_Z = NA_OFF_S + na_align(4 * NA_MAX_COLS);
 constexpr size_t NA_OFF_SP = NA_OFF_Z + na_align(4 * NA_MAX_COLS);
 constexpr size_t NA_OFF_DINV = NA_OFF_SP + na_align(4 * NA_MAX_COLS);
-constexpr size_t NA_BYTES = NA_OFF_DINV + na_align(4 * NA_MAX_COLS);   // 32.2 MiB
+constexpr size_t NA_OFF_R32 = NA_OFF_DINV + na_align(4 * NA_MAX_COLS);
+constexpr size_t NA_BYTES = NA_OFF_R32 + na_align(4 * NA_MAX_ROWS);   // 36.1 MiB
 constexpr int NA_MAX_DEVICES = 64;
 
 std::mutex g_scratch_mu;
@@ -550,6 +591,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     STP_HIP(grow(&h->z, &h->c_z, cols));
     STP_HIP(grow(&h->sp, &h->c_sp, cols));
     STP_HIP(grow(&h->dinv, &h->c_dinv, cols));
+    STP_HIP(grow(&h->r32, &h->c_r32, rows));
     STP_HIP(grow(&h->part0, &h->c_p0, nbq));
     STP_HIP(grow(&h->part1, &h->c_p1, nbc));
     STP_HIP(grow(&h->part2, &h->c_p2, nbs));
@@ -558,7 +600,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     STP_HIP(grow(&h->part5, &h->c_p5, nbc));
     STP_HIP(grow(&h->sc, &h->c_sc, 1));
     CglsWork w;
-    w.r = h->r; w.p = h->p; w.q = h->q; w.s = h->s; w.z = h->z; w.sp = h->sp; w.dinv = h->dinv;
+    w.r = h->r; w.p = h->p; w.q = h->q; w.s = h->s; w.z = h->z; w.sp = h->sp; w.dinv = h->dinv; w.r32 = h->r32;
     w.part0 = h->part0; w.part1 = h->part1; w.part2 = h->part2; w.part3 = h->part3; w.part4 = h->part4;
     w.part5 = h->part5; w.sc = h->sc;
     return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, st, w);
@@ -593,6 +635,7 @@ int stp_cgls_noalloc(const stp_csr* hc, const double* b, double lam, double tol,
     w.z = reinterpret_cast<float*>(base + NA_OFF_Z);
     w.sp = reinterpret_cast<float*>(base + NA_OFF_SP);
     w.dinv = reinterpret_cast<float*>(base + NA_OFF_DINV);
+    w.r32 = reinterpret_cast<float*>(base + NA_OFF_R32);
     return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, S(stream), w);
 }
 

@@ -15,17 +15,32 @@ struct SolverScalars {
 // upper bound of the grid (and so of the per-block partial sums) of the SpMV
 // and normal-residual kernels: they stride over the rows
 constexpr int SPMV_MAX_BLOCKS = 2048;
+// entries of the LDS staging buffer of the staged A^T product (FP32: 32 KB)
+constexpr int STAGE_CAP = 8192;
 
 int spmv_lanes(double mean_nnz, int transpose);
 int spmv_lanes_layout(double mean_nnz, int transpose);
 int spmv_blocks(int rows, int W);
+// rev != 0: rows visited last to first (same per-row results)
 hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
-                    double* part, const int* done, hipStream_t st);
+                    double* part, const int* done, hipStream_t st, int rev = 0);
 hipError_t spmv_f64(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, double* y,
                     double alpha, const double* wv, double lam, double* part, const int* done, hipStream_t st);
 hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
                            const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
-                           double* part_ss, const int* done, hipStream_t st);
+                           double* part_ss, const int* done, hipStream_t st, int rev = 0);
+// the same with r gathered in FP32 (FP64 accumulation)
+hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, const float* val, const float* r,
+                               const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
+                               double* part_ss, const int* done, int rev, hipStream_t st);
+// the same from the staged A^T (tiles of rt rows; per tile its distinct
+// entries of r are gathered once into LDS); the grid is
+// staged_blocks(ntiles), which is also the count of partials
+int staged_blocks(int ntiles);
+hipError_t normal_residual_staged(int W, int rows, int rt, int ntiles, int maxu, const int* rp, const uint16_t* l16,
+                                  const float* val, const int* tu, const int* uniq, const float* r, const double* x,
+                                  double lam, const float* dinv, float* s, float* z, double* part_sz,
+                                  double* part_ss, const int* done, int rev, hipStream_t st);
 hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
                     double* r, hipStream_t st);
 // FP64-valued variants (the square CG drop-in when FP64 values are attached)
@@ -48,10 +63,10 @@ hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const i
 // elementwise CGLS kernels (grid-stride), and so the count of |p|^2 partials.
 int cgls_blocks(int n);
 // alpha = gam[it&1] / (|q|^2 + lam |p|^2) (every block sums the partials),
-// x += alpha p, r -= alpha q
+// x += alpha p, r -= alpha q, and r32 = (float)r when r32 is given
 hipError_t launch_cgls_alpha_update(SolverScalars* sc, const double* pq, int nq, const double* pp, int np,
                                     double lam, int it, int cols, double* x, const float* p, int rows, double* r,
-                                    const float* q, hipStream_t st);
+                                    const float* q, float* r32, hipStream_t st);
 // rel, convergence test, beta = (z.s or z.(s - s_prev)) / gam[it&1] (every
 // block), gam[(it+1)&1] = z.s, then p = z + beta p with |p|^2 partials and
 // s_prev <- s; with first = 1 (a start or restart) beta = 0 and no scalar is

@@ -34,8 +34,8 @@ constexpr int MAX_BLOCKS = SPMV_MAX_BLOCKS;
 // instead of a chain of dependent load pairs.  Slots past the row end load
 // the row's first entry and contribute an exact zero, which leaves the sum
 // (and its order) unchanged.
-template <int W, typename AT, typename XT, typename VT = float, bool NOVAL = false>
-__device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const int* __restrict__ ci,
+template <int W, typename AT, typename XT, typename VT = float, bool NOVAL = false, typename IT = int>
+__device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const IT* __restrict__ ci,
                                       const VT* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
     constexpr int U = (128 / W) < 1 ? 1 : (128 / W);
     AT acc = 0;
@@ -80,14 +80,15 @@ template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART, 
 __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
                                                     const VT* __restrict__ val, const XT* __restrict__ x,
                                                     YT* __restrict__ y, double alpha, const double* __restrict__ wv,
-                                                    double lam, double* __restrict__ part, const int* done) {
+                                                    double lam, double* __restrict__ part, const int* done,
+                                                    int rev) {
     if (done && *done) return;
     constexpr int RPB = 256 / W;
     const int lane = threadIdx.x % W, sub = threadIdx.x / W;
     double ps = 0.0;
     for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
-        const int row = row0 + sub;
-        if (row < rows) {
+        if (row0 + sub < rows) {
+            const int row = rev ? rows - 1 - (row0 + sub) : row0 + sub;
             const AT acc = row_dot<W, AT, XT, VT>(rp, ci, val, x, row, lane);
             if (lane == 0) {
                 double v = alpha * (double)acc;
@@ -103,11 +104,11 @@ __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restri
 
 template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART, typename VT = float>
 static hipError_t spmv_w(int rows, const int* rp, const int* ci, const VT* val, const XT* x, YT* y, double alpha,
-                         const double* wv, double lam, double* part, const int* done, hipStream_t st) {
+                         const double* wv, double lam, double* part, const int* done, hipStream_t st, int rev = 0) {
     const int nb = spmv_blocks(rows, W);
     if (rows == 0) return hipSuccess;
     hipLaunchKernelGGL((spmv_kernel<W, XT, AT, YT, FUSE_W, PART, VT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val,
-                       x, y, alpha, wv, lam, part, done);
+                       x, y, alpha, wv, lam, part, done, rev);
     return hipGetLastError();
 }
 
@@ -147,14 +148,14 @@ int spmv_blocks(int rows, int W) {
 
 #define STP_SPMV_DISPATCH(XT, AT, YT, FW, PT)                                                                   \
     switch (W) {                                                                                                \
-        case 64: return spmv_w<64, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
-        case 32: return spmv_w<32, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
-        case 16: return spmv_w<16, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st); \
-        default: return spmv_w<8, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st);  \
+        case 64: return spmv_w<64, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st, rev); \
+        case 32: return spmv_w<32, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st, rev); \
+        case 16: return spmv_w<16, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st, rev); \
+        default: return spmv_w<8, XT, AT, YT, FW, PT>(rows, rp, ci, val, x, y, alpha, wv, lam, part, done, st, rev);  \
     }
 
 hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* val, const float* x, float* y,
-                    double* part, const int* done, hipStream_t st) {
+                    double* part, const int* done, hipStream_t st, int rev) {
     const double alpha = 1.0, lam = 0.0;
     const double* wv = nullptr;
     if (part) { STP_SPMV_DISPATCH(float, float, float, false, true) }
@@ -163,6 +164,7 @@ hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* 
 
 hipError_t spmv_f64(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, double* y,
                     double alpha, const double* wv, double lam, double* part, const int* done, hipStream_t st) {
+    const int rev = 0;
     if (wv) {
         if (part) { STP_SPMV_DISPATCH(double, double, double, true, true) }
         STP_SPMV_DISPATCH(double, double, double, true, false)
@@ -223,23 +225,25 @@ hipError_t gather_probe(int W, int rows, const int* rp, const int* ci, const XT*
 template hipError_t gather_probe<float>(int, int, const int*, const int*, const float*, float*, hipStream_t);
 template hipError_t gather_probe<double>(int, int, const int*, const int*, const double*, double*, hipStream_t);
 
-// s = A^T r - lam*x with FP64 r and x, FP32 output s, z = dinv*s, partials of s*z and s*s
-template <int W>
+// s = A^T r - lam x (FP64 accumulation; r gathered as RT: FP32 inside the
+// CGLS iteration, FP64 for the true residual), FP32 output s, z = dinv*s,
+// partials of s*z and s*s.  rev != 0 visits the rows last to first.
+template <int W, typename RT>
 __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const int* __restrict__ rp,
                                                                const int* __restrict__ ci, const float* __restrict__ val,
-                                                               const double* __restrict__ r,
+                                                               const RT* __restrict__ r,
                                                                const double* __restrict__ x, double lam,
                                                                const float* __restrict__ dinv, float* __restrict__ s,
                                                                float* __restrict__ z, double* __restrict__ part_sz,
-                                                               double* __restrict__ part_ss, const int* done) {
+                                                               double* __restrict__ part_ss, const int* done, int rev) {
     if (done && *done) return;
     constexpr int RPB = 256 / W;
     const int lane = threadIdx.x % W, sub = threadIdx.x / W;
     double psz = 0.0, pss = 0.0;
     for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
-        const int row = row0 + sub;
-        if (row < rows) {
-            const double acc = row_dot<W, double, double>(rp, ci, val, r, row, lane);
+        if (row0 + sub < rows) {
+            const int row = rev ? rows - 1 - (row0 + sub) : row0 + sub;
+            const double acc = row_dot<W, double, RT>(rp, ci, val, r, row, lane);
             if (lane == 0) {
                 const float sv = (float)(acc - lam * x[row]);
                 const float zv = dinv ? dinv[row] * sv : sv;
@@ -254,20 +258,96 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
     block_partial(pss, part_ss);
 }
 
-hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
-                           const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
-                           double* part_ss, const int* done, hipStream_t st) {
+template <typename RT>
+static hipError_t normal_residual_t(int W, int rows, const int* rp, const int* ci, const float* val, const RT* r,
+                                    const double* x, double lam, const float* dinv, float* s, float* z,
+                                    double* part_sz, double* part_ss, const int* done, int rev, hipStream_t st) {
     const int nb = spmv_blocks(rows, W);
     if (rows == 0) return hipSuccess;
     switch (W) {
-        case 64: hipLaunchKernelGGL(normal_residual_kernel<64>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
-                                    lam, dinv, s, z, part_sz, part_ss, done); break;
-        case 32: hipLaunchKernelGGL(normal_residual_kernel<32>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
-                                    lam, dinv, s, z, part_sz, part_ss, done); break;
-        case 16: hipLaunchKernelGGL(normal_residual_kernel<16>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
-                                    lam, dinv, s, z, part_sz, part_ss, done); break;
-        default: hipLaunchKernelGGL(normal_residual_kernel<8>, dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r, x,
-                                    lam, dinv, s, z, part_sz, part_ss, done); break;
+        case 64: hipLaunchKernelGGL((normal_residual_kernel<64, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        case 32: hipLaunchKernelGGL((normal_residual_kernel<32, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        case 16: hipLaunchKernelGGL((normal_residual_kernel<16, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        default: hipLaunchKernelGGL((normal_residual_kernel<8, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
+                           const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
+                           double* part_ss, const int* done, hipStream_t st, int rev) {
+    return normal_residual_t<double>(W, rows, rp, ci, val, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev, st);
+}
+
+hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, const float* val, const float* r,
+                               const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
+                               double* part_ss, const int* done, int rev, hipStream_t st) {
+    return normal_residual_t<float>(W, rows, rp, ci, val, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev, st);
+}
+
+// Staged A^T r: one workgroup per tile of rt rows (cells) at a time.  The
+// tile's distinct rays are read from r once, into LDS (uniq is sorted, so
+// neighbouring lanes read neighbouring rays), and the rows' dot products then
+// gather from LDS through 16-bit positions; the row order, lanes and chunking
+// are those of normal_residual_kernel, so each row's sum is the same.
+template <int W>
+__global__ void __launch_bounds__(256) normal_residual_staged_kernel(
+    int rows, int rt, int ntiles, const int* __restrict__ rp, const uint16_t* __restrict__ l16,
+    const float* __restrict__ val, const int* __restrict__ tu, const int* __restrict__ uniq,
+    const float* __restrict__ r, const double* __restrict__ x, double lam, const float* __restrict__ dinv,
+    float* __restrict__ s, float* __restrict__ z, double* __restrict__ part_sz, double* __restrict__ part_ss,
+    const int* done, int rev) {
+    extern __shared__ float xs[];
+    if (done && *done) return;
+    constexpr int RPB = 256 / W;
+    const int lane = threadIdx.x % W, sub = threadIdx.x / W;
+    double psz = 0.0, pss = 0.0;
+    for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
+        const int t = rev ? ntiles - 1 - t0 : t0;
+        const int ub = tu[t], nu = tu[t + 1] - ub;
+        __syncthreads();   // the previous tile's reads of xs are done
+        for (int k = threadIdx.x; k < nu; k += 256) xs[k] = r[uniq[ub + k]];
+        __syncthreads();
+        const int r1 = min(t * rt + rt, rows);
+        for (int row = t * rt + sub; row < r1; row += RPB) {
+            const double acc = row_dot<W, double, float, float, false, uint16_t>(rp, l16, val, xs, row, lane);
+            if (lane == 0) {
+                const float sv = (float)(acc - lam * x[row]);
+                const float zv = dinv ? dinv[row] * sv : sv;
+                s[row] = sv;
+                z[row] = zv;
+                psz += (double)sv * (double)zv;
+                pss += (double)sv * (double)sv;
+            }
+        }
+    }
+    block_partial(psz, part_sz);
+    block_partial(pss, part_ss);
+}
+
+int staged_blocks(int ntiles) { return ntiles < 1 ? 1 : (ntiles > MAX_BLOCKS ? MAX_BLOCKS : ntiles); }
+
+hipError_t normal_residual_staged(int W, int rows, int rt, int ntiles, int maxu, const int* rp, const uint16_t* l16,
+                                  const float* val, const int* tu, const int* uniq, const float* r, const double* x,
+                                  double lam, const float* dinv, float* s, float* z, double* part_sz,
+                                  double* part_ss, const int* done, int rev, hipStream_t st) {
+    if (rows == 0) return hipSuccess;
+    if (maxu < 0 || maxu > STAGE_CAP || rt < 1) return hipErrorInvalidValue;
+    const int nb = staged_blocks(ntiles);
+    const size_t lds = sizeof(float) * (size_t)(maxu > 0 ? maxu : 1);
+    switch (W) {
+        case 64: hipLaunchKernelGGL(normal_residual_staged_kernel<64>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
+                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        case 32: hipLaunchKernelGGL(normal_residual_staged_kernel<32>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
+                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        case 16: hipLaunchKernelGGL(normal_residual_staged_kernel<16>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
+                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        default: hipLaunchKernelGGL(normal_residual_staged_kernel<8>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
+                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
     }
     return hipGetLastError();
 }
@@ -299,7 +379,8 @@ __global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* s
                                                                 const double* pp, int np, double lam, int it,
                                                                 int cols, double* __restrict__ x,
                                                                 const float* __restrict__ p, int rows,
-                                                                double* __restrict__ r, const float* __restrict__ q) {
+                                                                double* __restrict__ r, const float* __restrict__ q,
+                                                                float* __restrict__ r32) {
     __shared__ int s_done;
     if (threadIdx.x == 0) s_done = sc->done;
     __syncthreads();
@@ -317,7 +398,11 @@ __global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* s
     const int n = cols > rows ? cols : rows;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < cols) x[i] += a * (double)p[i];
-        if (i < rows) r[i] -= a * (double)q[i];
+        if (i < rows) {
+            const double rv = r[i] - a * (double)q[i];
+            r[i] = rv;
+            if (r32) r32[i] = (float)rv;
+        }
     }
 }
 
@@ -795,9 +880,9 @@ int cgls_blocks(int n) {
 }
 hipError_t launch_cgls_alpha_update(SolverScalars* sc, const double* pq, int nq, const double* pp, int np,
                                     double lam, int it, int cols, double* x, const float* p, int rows, double* r,
-                                    const float* q, hipStream_t st) {
+                                    const float* q, float* r32, hipStream_t st) {
     hipLaunchKernelGGL(cgls_alpha_update_kernel, dim3(cgls_blocks(cols > rows ? cols : rows)), dim3(256), 0, st, sc,
-                       pq, nq, pp, np, lam, it, cols, x, p, rows, r, q);
+                       pq, nq, pp, np, lam, it, cols, x, p, rows, r, q, r32);
     return hipGetLastError();
 }
 hipError_t launch_cgls_beta_direction(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
