@@ -28,8 +28,14 @@ struct ConvShape {
 // out = act(conv(in, w)); relu applies max(0, .) before the store.
 hipError_t conv3d_fwd(const ConvShape& s, const bf16_t* in, const bf16_t* w, int relu, int out_kind, void* out,
                       int out_ld, int out_off, hipStream_t st);
-// dw[co][t][ci] += sum_p dout[p][co] * in[p + d_t][ci]  (FP32 accumulate into dw)
-hipError_t conv3d_bwd_weight(const ConvShape& s, const bf16_t* in, const bf16_t* dout, float* dw, hipStream_t st);
+// dw[co][t][ci] = sum_p dout[p][co] * in[p + d_t][ci]: FP32 partial sums of
+// voxel chunks in `work` (conv3d_bwd_weight_work(s) floats), then summed
+// over the chunks in chunk order, so the result does not depend on timing
+size_t conv3d_bwd_weight_work(const ConvShape& s);
+// an upper bound of conv3d_bwd_weight_work over all shapes
+constexpr size_t CONV_BWD_WORK_MAX = (size_t)8192 * 256;
+hipError_t conv3d_bwd_weight(const ConvShape& s, const bf16_t* in, const bf16_t* dout, float* dw, float* work,
+                             hipStream_t st);
 // weights of the data-gradient convolution: wb[ci][t][co] = w[co][26 - t][ci]
 hipError_t conv3d_flip_weights(int cin, int cout, const float* w32, bf16_t* wb, hipStream_t st);
 

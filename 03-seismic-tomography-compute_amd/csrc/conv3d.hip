@@ -14,7 +14,8 @@
 //
 // The weight gradient is the GEMM dw[co][(t,ci)] = sum_p dout[p][co] *
 // in[p + d_t][ci] with K = voxels: a wavefront owns one 16x16 block of dw and
-// a range of voxels, and adds its partial block into dw with FP32 atomics.
+// a chunk of voxels and writes its partial block to a workspace; a second
+// kernel sums the chunks in chunk order, so training is reproducible.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -111,10 +112,12 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(int n0, int n1, int n2, i
 }
 
 // One wavefront per (16 x 16 block of dw, voxel chunk).  blockIdx.x: 4 dw
-// blocks (one per wavefront), blockIdx.y: voxel chunk of `vc` voxels.
+// blocks (one per wavefront), blockIdx.y: voxel chunk of `vc` voxels; each
+// writes its chunk's partial block to work[chunk][co][t][ci].
 __global__ void __launch_bounds__(256) conv_bwd_weight_kernel(int n0, int n1, int n2, int cin, int cout,
                                                               const bf16_t* __restrict__ in,
-                                                              const bf16_t* __restrict__ dout, float* dw, int vc) {
+                                                              const bf16_t* __restrict__ dout,
+                                                              float* __restrict__ work, int vc) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ncolb = (27 * cin) >> 4;
@@ -154,10 +157,20 @@ __global__ void __launch_bounds__(256) conv_bwd_weight_kernel(int n0, int n1, in
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc, 0, 0, 0);
     }
     // D[4*kg + i][m]: row co' = mb*16 + 4*kg + i, column cb*16 + m
+    float* part = work + (size_t)blockIdx.y * cout * 27 * cin;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int cr = mb * 16 + 4 * kg + i;
-        atomicAdd(dw + (size_t)cr * 27 * cin + cb * 16 + m, acc[i]);
+        part[(size_t)cr * 27 * cin + cb * 16 + m] = acc[i];
+    }
+}
+
+// dw[e] = sum over chunks c (in order) of work[c][e]
+__global__ void sum_chunks_kernel(int64_t n, int chunks, const float* __restrict__ work, float* __restrict__ dw) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = 0.f;
+        for (int c = 0; c < chunks; ++c) v += work[(size_t)c * n + i];
+        dw[i] = v;
     }
 }
 
@@ -323,20 +336,40 @@ hipError_t conv3d_fwd(const ConvShape& s, const bf16_t* in, const bf16_t* w, int
     return hipGetLastError();
 }
 
-hipError_t conv3d_bwd_weight(const ConvShape& s, const bf16_t* in, const bf16_t* dout, float* dw, hipStream_t st) {
-    if ((s.cin & 15) || (s.cout & 15)) return hipErrorInvalidValue;
+// voxel chunks of the weight gradient: about 8192 wavefronts in total, at
+// least 512 voxels (a multiple of 32) each
+static void bwd_weight_chunks(const ConvShape& s, int* chunks_out, int* vc_out) {
     const int nvox = s.n0 * s.n1 * s.n2;
-    if (nvox <= 0) return hipSuccess;
     const int nblk = (s.cout / 16) * (27 * s.cin / 16);
-    // voxel chunks: about 8192 wavefronts in total, at least 512 voxels each
     int chunks = 8192 / nblk;
     if (chunks < 1) chunks = 1;
     int vc = (nvox + chunks - 1) / chunks;
     vc = ((vc + 31) / 32) * 32;
     if (vc < 512) vc = 512;
-    chunks = (nvox + vc - 1) / vc;
+    *chunks_out = (nvox + vc - 1) / vc;
+    *vc_out = vc;
+}
+
+size_t conv3d_bwd_weight_work(const ConvShape& s) {
+    if (s.n0 * s.n1 * s.n2 <= 0 || s.cin < 16 || s.cout < 16) return 0;
+    int chunks = 0, vc = 0;
+    bwd_weight_chunks(s, &chunks, &vc);
+    return (size_t)chunks * s.cout * 27 * s.cin;
+}
+
+hipError_t conv3d_bwd_weight(const ConvShape& s, const bf16_t* in, const bf16_t* dout, float* dw, float* work,
+                             hipStream_t st) {
+    if ((s.cin & 15) || (s.cout & 15)) return hipErrorInvalidValue;
+    const int nvox = s.n0 * s.n1 * s.n2;
+    if (nvox <= 0) return hipSuccess;
+    if (conv3d_bwd_weight_work(s) > CONV_BWD_WORK_MAX) return hipErrorInvalidValue;
+    const int nblk = (s.cout / 16) * (27 * s.cin / 16);
+    int chunks = 0, vc = 0;
+    bwd_weight_chunks(s, &chunks, &vc);
     hipLaunchKernelGGL(conv_bwd_weight_kernel, dim3((nblk + 3) / 4, chunks), dim3(256), 0, st, s.n0, s.n1, s.n2, s.cin,
-                       s.cout, in, dout, dw, vc);
+                       s.cout, in, dout, work, vc);
+    const int64_t n = (int64_t)s.cout * 27 * s.cin;
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3(grid_for(n)), dim3(256), 0, st, n, chunks, work, dw);
     return hipGetLastError();
 }
 

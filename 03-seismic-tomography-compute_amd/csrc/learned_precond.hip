@@ -337,6 +337,7 @@ struct stp_lpc {
     bf16_t* wbf = nullptr;
     bf16_t* wflip = nullptr;
     float *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    float* wdw = nullptr;   // per-chunk partials of the weight gradient (CONV_BWD_WORK_MAX)
     int64_t steps = 0;
     // grid-sized workspace
     int n0 = 0, n1 = 0, n2 = 0;
@@ -377,7 +378,7 @@ void free_grid(stp_lpc* h) {
 
 void free_all(stp_lpc* h) {
     free_grid(h);
-    void* ps[] = {h->w32, h->wbf, h->wflip, h->grad, h->m1, h->m2};
+    void* ps[] = {h->w32, h->wbf, h->wflip, h->grad, h->m1, h->m2, h->wdw};
     for (void* p : ps)
         if (p) hipFree(p);
 }
@@ -492,31 +493,31 @@ int net_backward(stp_lpc* h, hipStream_t st) {
     int rc;
     STP_HIP(hipMemsetAsync(h->grad, 0, sizeof(float) * total_params(), st));
     // o
-    STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, LO), h->e, h->dO, G(LO), st));
+    STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, LO), h->e, h->dO, G(LO), h->wdw, st));
     if ((rc = conv_bwd_data(h, LO, a0, a1, a2, h->dO, h->de, st)) != STP_OK) return rc;
     STP_HIP(relu_mask_bf16(V * 16, h->de, h->e, h->dpre, st));
     // d1
-    STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, D1), h->cat1, h->dpre, G(D1), st));
+    STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, D1), h->cat1, h->dpre, G(D1), h->wdw, st));
     if ((rc = conv_bwd_data(h, D1, a0, a1, a2, h->dpre, h->dcat1, st)) != STP_OK) return rc;
     STP_HIP(upcat_bwd(a0, a1, a2, 32, 16, h->dcat1, h->dd, h->da, st));
     STP_HIP(relu_mask_bf16(V2 * 32, h->dd, h->d, h->dpre, st));
     // d2
-    STP_HIP(conv3d_bwd_weight(shp(b0, b1, b2, D2), h->cat2, h->dpre, G(D2), st));
+    STP_HIP(conv3d_bwd_weight(shp(b0, b1, b2, D2), h->cat2, h->dpre, G(D2), h->wdw, st));
     if ((rc = conv_bwd_data(h, D2, b0, b1, b2, h->dpre, h->dcat2, st)) != STP_OK) return rc;
     STP_HIP(upcat_bwd(b0, b1, b2, 64, 32, h->dcat2, h->dc, h->db, st));
     STP_HIP(relu_mask_bf16(V3 * 64, h->dc, h->c, h->dpre, st));
     // e3
-    STP_HIP(conv3d_bwd_weight(shp(c0, c1, c2, E3), h->pb, h->dpre, G(E3), st));
+    STP_HIP(conv3d_bwd_weight(shp(c0, c1, c2, E3), h->pb, h->dpre, G(E3), h->wdw, st));
     if ((rc = conv_bwd_data(h, E3, c0, c1, c2, h->dpre, h->dpb, st)) != STP_OK) return rc;
     STP_HIP(avgpool2_bwd(b0, b1, b2, 32, h->dpb, h->db, st));
     STP_HIP(relu_mask_bf16(V2 * 32, h->db, h->b, h->dpre, st));
     // e2
-    STP_HIP(conv3d_bwd_weight(shp(b0, b1, b2, E2), h->pa, h->dpre, G(E2), st));
+    STP_HIP(conv3d_bwd_weight(shp(b0, b1, b2, E2), h->pa, h->dpre, G(E2), h->wdw, st));
     if ((rc = conv_bwd_data(h, E2, b0, b1, b2, h->dpre, h->dpa, st)) != STP_OK) return rc;
     STP_HIP(avgpool2_bwd(a0, a1, a2, 16, h->dpa, h->da, st));
     STP_HIP(relu_mask_bf16(V * 16, h->da, h->a, h->dpre, st));
     // e1
-    STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, E1), h->feat, h->dpre, G(E1), st));
+    STP_HIP(conv3d_bwd_weight(shp(a0, a1, a2, E1), h->feat, h->dpre, G(E1), h->wdw, st));
     return STP_OK;
 }
 
@@ -558,7 +559,8 @@ int stp_lpc_create(uint64_t seed, stp_lpc** out) {
     hipError_t e;
     if ((e = dalloc(&h->w32, P)) != hipSuccess || (e = dalloc(&h->wbf, P)) != hipSuccess ||
         (e = dalloc(&h->wflip, maxl)) != hipSuccess || (e = dalloc(&h->grad, P)) != hipSuccess ||
-        (e = dalloc(&h->m1, P)) != hipSuccess || (e = dalloc(&h->m2, P)) != hipSuccess) {
+        (e = dalloc(&h->m1, P)) != hipSuccess || (e = dalloc(&h->m2, P)) != hipSuccess ||
+        (e = dalloc(&h->wdw, stp::CONV_BWD_WORK_MAX)) != hipSuccess) {
         free_all(h);
         delete h;
         return stp::hip_fail(e, "stp_lpc_create");
@@ -743,8 +745,11 @@ int stp_conv3d_bf16_bwd(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t
     const stp::ConvShape s{(int)n0, (int)n1, (int)n2, cin, cout};
     hipError_t e = hipSuccess;
     if (dw_dev) {
-        e = hipMemsetAsync(dw_dev, 0, sizeof(float) * cin * cout * NTAP, st);
-        if (e == hipSuccess) e = stp::conv3d_bwd_weight(s, in_dev, dout_dev, dw_dev, st);
+        float* work = nullptr;
+        e = dalloc(&work, std::max<size_t>(stp::conv3d_bwd_weight_work(s), 1));
+        if (e == hipSuccess) e = stp::conv3d_bwd_weight(s, in_dev, dout_dev, dw_dev, work, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (work) hipFree(work);
     }
     if (e == hipSuccess && din_dev) {
         bf16_t* wb = nullptr;

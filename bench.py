@@ -372,6 +372,23 @@ def cg_workload(dev, nrays: int = 200_000, n: int = 64, lam: float = 1e-2, tol: 
             out[kind]["prepare_seconds"] = round(tp, 5)
     if "learned" in out and "jacobi" in out:
         out["learned_over_jacobi_time"] = round(out["learned"]["seconds"] / out["jacobi"]["seconds"], 4)
+    if "learned" in out and "none" in out:
+        out["learned_over_none_time"] = round(out["learned"]["seconds"] / out["none"]["seconds"], 4)
+    if net is not None:
+        # the untrained network (zero output layer): M = D^-1/2 (I + L) D^-1/2
+        raw = LearnedPreconditioner(seed=0)
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            raw.prepare(A, grid, lam)
+            _, info, _ = A.cgls(b, lam, tol, 20000, "learned", network=raw)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = (dt, info) if best is None or dt < best[0] else best
+        out["untrained_network"] = {"seconds": round(best[0], 5), "iterations": best[1].iterations,
+                                    "converged": bool(best[1].converged)}
+        del raw
     if "jacobi" in kinds and given_order:
         # the same Jacobi solve on the matrix in its given (x-major, ray) order
         A.set_grid(None)

@@ -166,31 +166,30 @@ def test_learned_cgls_falls_back_to_jacobi(cuda):
     assert info.iterations == ij.iterations
 
 
-def test_training_runs_and_selection_keeps_the_better_network(cuda):
+def test_training_is_deterministic_and_moves_the_network(cuda):
     """A short run of the training recipe at 32^3 (systems of the 200k-ray
-    64^3 density per cell): every step gives a finite loss and moves the
-    parameters, and the network kept by validation (the recipe's rule: the
-    trained weights only if they need fewer held-out iterations than the
-    weights before) converges to the Jacobi solution in fewer CGLS iterations
-    than Jacobi and in no more than the untrained network.  Training gradients
-    are checked against the oracle in test_training_loss_and_gradient_match_oracle;
-    whether a given run lowers the held-out loss depends on the atomic order of
-    the BF16 convolution gradients, so it is not asserted here."""
-    net = LearnedPreconditioner(seed=2)
+    64^3 density per cell, two data vectors each): every step gives a finite
+    loss and moves the parameters, and a second run from the same seed on the
+    same systems gives the same parameters bit for bit (the weight gradient
+    sums its voxel chunks in a fixed order).  Whether training lowers the
+    held-out iteration count is measured, not asserted: on this recipe it
+    does not, beyond the few-iteration noise of the count itself."""
+    systems = [make_system((32, 32, 32), 50_000, seed=100 + 2 * i, n_rhs=2) for i in range(6)]
     held = make_system((32, 32, 32), 50_000, seed=41)
-    ij, il0 = solve_iterations(net, held, LAM)
-    p0 = net.get_params()
-    log = train(net, (make_system((32, 32, 32), 50_000, seed=100 + 2 * i) for i in range(12)), LAM, (20, 60), 1,
-                1e-4, 3e-5, n_steps=12)
-    assert log.steps == 12 and all(np.isfinite(log.losses))
-    assert float(np.abs(net.get_params() - p0).max()) > 0
+    runs = []
+    for _ in range(2):
+        net = LearnedPreconditioner(seed=2)
+        p0 = net.get_params()
+        log = train(net, systems, LAM, (20, 60), 1, 1e-4, 3e-5)
+        assert log.steps == 6 and all(np.isfinite(log.losses))
+        assert float(np.abs(net.get_params() - p0).max()) > 0
+        runs.append((net.get_params(), log.losses))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    ij, il0 = solve_iterations(LearnedPreconditioner(seed=2), held, LAM)
     _, il1 = solve_iterations(net, held, LAM)
-    if il1 >= il0:
-        net.set_params(p0)
-    _, il = solve_iterations(net, held, LAM)
-    print(f"held-out 32^3: CGLS iterations Jacobi {ij}, untrained {il0}, trained {il1}, kept {il}", flush=True)
-    assert il == min(il0, il1)
-    assert il < ij
+    print(f"held-out 32^3: CGLS iterations Jacobi {ij}, untrained {il0}, after 6 steps {il1}", flush=True)
+    assert il0 < ij
     x_l, info_l, _ = held.A.cgls(held.b, LAM, 1e-6, 5000, "learned", network=net)
     x_j, info_j, _ = held.A.cgls(held.b, LAM, 1e-6, 5000, "jacobi")
     assert info_l.converged and not info_l.fallback and info_j.converged
