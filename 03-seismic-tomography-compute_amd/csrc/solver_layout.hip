@@ -124,22 +124,24 @@ __global__ void stage_key_kernel(int rows, const int* __restrict__ rp, const int
 }
 
 // after the sort: flag the first entry of each distinct key (flag[nnz] = 0)
-// and count the distinct keys of each tile
-__global__ void stage_flag_kernel(int nnz, const uint64_t* __restrict__ key, int* __restrict__ flag,
-                                  int* __restrict__ cnt) {
+__global__ void stage_flag_kernel(int nnz, const uint64_t* __restrict__ key, int* __restrict__ flag) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > nnz) return;
-    if (j == nnz) {
-        flag[j] = 0;
-        return;
-    }
-    const int f = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
-    flag[j] = f;
-    if (f) atomicAdd(&cnt[key[j] >> 32], 1);
+    flag[j] = (j < nnz && (j == 0 || key[j] != key[j - 1])) ? 1 : 0;
 }
 
-// uidx = exclusive scan of flag, tu = exclusive scan of the tile counts: the
-// entry's distinct key is number uidx + flag - 1, its tile's list starts at tu
+// uidx = exclusive scan of flag.  The sorted entries of tile t start at
+// rp[t * rt] (the entries of the tiles before it), so its list starts at
+// tu[t] = uidx[rp[min(t * rt, rows)]]; tu[nt] is the total.
+__global__ void stage_tiles_kernel(int nt, int rt, int rows, const int* __restrict__ rp, const int* __restrict__ uidx,
+                                   int* __restrict__ tu) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > nt) return;
+    const int64_t r = (int64_t)t * rt < rows ? (int64_t)t * rt : rows;
+    tu[t] = uidx[rp[r]];
+}
+
+// the entry's distinct key is number uidx + flag - 1, its tile's list starts at tu
 __global__ void stage_fill_kernel(int nnz, const uint64_t* __restrict__ key, const int* __restrict__ flag,
                                   const int* __restrict__ uidx, const int* __restrict__ idx,
                                   const int* __restrict__ tu, int* __restrict__ uniq, uint16_t* __restrict__ l16) {
@@ -175,7 +177,7 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
     const int nnz = (int)h->nnz;
     if (n == 0 || nnz == 0) return hipSuccess;
     uint64_t *k0 = nullptr, *k1 = nullptr;
-    int *i0 = nullptr, *i1 = nullptr, *flag = nullptr, *uidx = nullptr, *cnt = nullptr, *tu = nullptr;
+    int *i0 = nullptr, *i1 = nullptr, *flag = nullptr, *uidx = nullptr, *tu = nullptr;
     hipError_t e = hipSuccess;
     do {
         if ((e = hipMalloc(&k0, sizeof(uint64_t) * nnz)) != hipSuccess) break;
@@ -186,27 +188,27 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
         if ((e = hipMalloc(&uidx, sizeof(int) * ((size_t)nnz + 1))) != hipSuccess) break;
         for (int rt = 256; rt >= 32; rt /= 2) {
             const int nt = (n + rt - 1) / rt;
-            hipFree(cnt);
             hipFree(tu);
-            cnt = tu = nullptr;
-            if ((e = hipMalloc(&cnt, sizeof(int) * ((size_t)nt + 1))) != hipSuccess) break;
+            tu = nullptr;
             if ((e = hipMalloc(&tu, sizeof(int) * ((size_t)nt + 1))) != hipSuccess) break;
-            if ((e = hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)nt + 1), st)) != hipSuccess) break;
             hipLaunchKernelGGL(stage_key_kernel, dim3(blocks((int64_t)n * 64)), dim3(256), 0, st, n, h->ltrp, h->ltci,
                                rt, k0, i0);
             if ((e = hipGetLastError()) != hipSuccess) break;
             if ((e = sort_pairs(k0, k1, i0, i1, nnz, 32 + bits_for((int64_t)nt), st)) != hipSuccess) break;
-            hipLaunchKernelGGL(stage_flag_kernel, dim3(blocks((int64_t)nnz + 1)), dim3(256), 0, st, nnz, k1, flag,
-                               cnt);
+            hipLaunchKernelGGL(stage_flag_kernel, dim3(blocks((int64_t)nnz + 1)), dim3(256), 0, st, nnz, k1, flag);
             if ((e = hipGetLastError()) != hipSuccess) break;
-            if ((e = exclusive_scan_i32(cnt, tu, nt + 1, st)) != hipSuccess) break;
+            if ((e = exclusive_scan_i32(flag, uidx, nnz + 1, st)) != hipSuccess) break;
+            hipLaunchKernelGGL(stage_tiles_kernel, dim3(blocks((int64_t)nt + 1)), dim3(256), 0, st, nt, rt, n, h->ltrp,
+                               uidx, tu);
+            if ((e = hipGetLastError()) != hipSuccess) break;
             std::vector<int> htu((size_t)nt + 1);
-            if ((e = hipMemcpy(htu.data(), tu, sizeof(int) * ((size_t)nt + 1), hipMemcpyDeviceToHost)) != hipSuccess)
+            if ((e = hipMemcpyAsync(htu.data(), tu, sizeof(int) * ((size_t)nt + 1), hipMemcpyDeviceToHost, st)) !=
+                hipSuccess)
                 break;
+            if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
             int maxu = 0;
             for (int t = 0; t < nt; ++t) maxu = std::max(maxu, htu[(size_t)t + 1] - htu[(size_t)t]);
             if (maxu > STAGE_CAP) continue;
-            if ((e = exclusive_scan_i32(flag, uidx, nnz + 1, st)) != hipSuccess) break;
             if ((e = hipMalloc(&h->st_uniq, sizeof(int) * (size_t)std::max(htu[(size_t)nt], 1))) != hipSuccess) break;
             if ((e = hipMalloc(&h->st_l16, sizeof(uint16_t) * (size_t)nnz)) != hipSuccess) break;
             hipLaunchKernelGGL(stage_fill_kernel, dim3(blocks(nnz)), dim3(256), 0, st, nnz, k1, flag, uidx, i1, tu,
@@ -221,7 +223,7 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     } while (0);
-    hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(flag); hipFree(uidx); hipFree(cnt); hipFree(tu);
+    hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(flag); hipFree(uidx); hipFree(tu);
     return e;
 }
 

@@ -11,9 +11,8 @@
 //   * RTMEngine::apply_image_condition / finalize_image / reset_image
 //     (src/cpp/rtm/rtm_engine.cpp:24-57, 170-190);
 //   * load_config (src/cpp/core/config.cpp:9-96);
-//   * CartesianGrid index math (src/cpp/core/grid.cpp:13-49);
 //   * TerrainBoundary (src/cpp/solver/boundary.cpp:8-204) on a single-block
-//     grid.
+//     CartesianGrid (src/cpp/core/grid.cpp:13-49).
 #include <cstdint>
 #include <cstring>
 #include <map>
@@ -131,13 +130,6 @@ int ref_load_config(const char* path, double* out, char* misfit_type, char* outp
     } catch (...) {
         return 1;
     }
-}
-
-int64_t ref_grid_index_with_halo(int64_t nx, int64_t ny, int64_t nz, int hw, int64_t ix, int64_t iy, int64_t iz) {
-    GridParams p{nx, ny, nz, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, hw};
-    const int coords[3] = {0, 0, 0}, dims[3] = {1, 1, 1};
-    CartesianGrid g(p, coords, dims);
-    return g.local_index_with_halo(ix, iy, iz);
 }
 
 // TerrainBoundary on an nx x ny x nz grid (halo hw, spacing h, origin o):
