@@ -17,9 +17,11 @@ struct stp_csr {
     double *r = nullptr, *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr;
     double *part4 = nullptr, *part5 = nullptr;
     float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr, *r32 = nullptr;
+    double *ry = nullptr, *rg = nullptr;   // CGLS: change of r (rows) and A^T r of the pass start (cols)
+    double* pnorm = nullptr;               // CGLS: |r|^2 and |y|^2 partials (2 x 256)
     double *dp = nullptr, *dap = nullptr, *dz = nullptr, *dr = nullptr, *ddinv = nullptr;
     stp::SolverScalars* sc = nullptr;
-    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0, c_r32 = 0;
+    size_t c_r = 0, c_p0 = 0, c_p1 = 0, c_p2 = 0, c_p3 = 0, c_p4 = 0, c_p5 = 0, c_p = 0, c_q = 0, c_s = 0, c_z = 0, c_sp = 0, c_dinv = 0, c_r32 = 0, c_ry = 0, c_rg = 0, c_pnorm = 0;
     // solver layout (solver_layout.hip): columns are the cells of a
     // gn[0]*gn[1]*gn[2] grid when gn[0] > 0; the renumbered copy is built on
     // the first CGLS solve or by stp_csr_prepare_layout

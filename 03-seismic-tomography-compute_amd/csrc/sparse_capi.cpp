@@ -48,7 +48,7 @@ hipError_t grow(T** p, size_t* cap, size_t n) {
 
 static void csr_free(stp_csr* h) {
     void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->val64, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
-                  h->p, h->q, h->s, h->z, h->sp, h->dinv, h->r32, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
+                  h->p, h->q, h->s, h->z, h->sp, h->dinv, h->r32, h->ry, h->rg, h->pnorm, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
     for (void* p : ps)
         if (p) hipFree(p);
     stp::free_solver_layout(h);
@@ -351,13 +351,15 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
 namespace {
 
 // Device workspace of one CGLS solve: r (rows, FP64), q (rows, FP32), p, s,
-// z, s_prev and the Jacobi diagonal (cols, FP32), the per-block partial sums
-// and the device-resident scalars.
+// z, s_prev and the Jacobi diagonal (cols, FP32), for a pass that gathers
+// FP32 the change y of r (rows, FP64), the FP32 copy of the gathered vector
+// (rows) and g = A^T r at the start of the pass (cols, FP64), the per-block
+// partial sums and the device-resident scalars.
 struct CglsWork {
-    double* r = nullptr;
-    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr, *r32 = nullptr;
+    double *r = nullptr, *y = nullptr, *g = nullptr;
+    float *p = nullptr, *q = nullptr, *s = nullptr, *z = nullptr, *sp = nullptr, *dinv = nullptr, *v32 = nullptr;
     double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *part4 = nullptr,
-           *part5 = nullptr;
+           *part5 = nullptr, *pnorm = nullptr;   // pnorm: |r|^2 and |y|^2 partials, 2 x 256
     stp::SolverScalars* sc = nullptr;
 };
 
@@ -404,11 +406,28 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
     }
     int* done = &w.sc->done;
     const size_t fw = sizeof(float) / 4, dw = sizeof(double) / 4;
+    // For tol >= R32_MIN_TOL the iteration's A^T r gathers an FP32 vector (in
+    // the solver layout from the staged A^T, whose tiles read their distinct
+    // rays once, into LDS; FP64 sums).  Its rounding error is relative to the
+    // vector gathered, so a pass starts with g - A^T y, g = A^T r_ref in FP64
+    // at the start of the pass and y = r_ref - r (the sum of the alpha q since
+    // then), and switches to A^T r itself once |y| exceeds |r| (checked every
+    // CHECK iterations): consistent data drive r to 0, data with a large part
+    // outside A's range keep |r| large while |y| stays bounded by the part in
+    // range.  r is updated in FP64 throughout.  A pass that stalls (no 10 %
+    // gain in STALL iterations) or ends above tol in the FP64 check restarts
+    // from the recomputed residual with r in FP64.  Odd iterations sweep the
+    // rows of both products last to first, so each product starts on the part
+    // of its matrix that the previous sweep left in the device cache.
+    bool use32 = tol >= R32_MIN_TOL;
+    const int npa = stp::cgls_blocks(cols > rows ? cols : rows);   // blocks of the alpha/update kernel
+    double hnorm[2 * 256];   // npa <= 256
 
-    // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s
+    // s = A^T r - lam x, z = M s, gamma = s.z, p = z, s_prev = s (and g, y = 0)
     auto start = [&](int set_s0, int it0) -> int {
         STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3, nullptr,
-                                     st));
+                                     st, 0, use32 ? w.g : nullptr));
+        if (use32) STP_HIP(stp::launch_zero_words(w.y, dw * rows, st));
         if (flex) {
             STP_HIP(stp::launch_zero_words(w.sp, fw * cols, st));
             int rc = stp::lpc_apply(precond_handle, w.s, w.z, nullptr, w.sp, w.part4, w.part5, st, perm);
@@ -433,35 +452,28 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
     int it = 0, replacements = 0;
     double true_rel = 0.0;
     double ps[stp::SPMV_MAX_BLOCKS];
-    // For tol >= R32_MIN_TOL the iteration's A^T r gathers r rounded to FP32
-    // (r itself stays FP64, the sum is FP64): in the solver layout from the
-    // staged A^T, whose tiles read their distinct rays from r once, into LDS.
-    // That rounding bounds the attainable residual near 1e-7 of |A^T b| on
-    // systems whose residual r stays large, so a pass that stalls (no 10 %
-    // gain in STALL iterations) or ends above tol in the FP64 check restarts
-    // from the recomputed residual with r gathered in FP64.  Odd iterations
-    // sweep the rows of both products last to first, so each product starts
-    // on the part of its matrix that the previous sweep left in the device
-    // cache.
-    bool use32 = tol >= R32_MIN_TOL;
     const bool staged = lay && h->st_rt > 0;
     for (;;) {
         const int nbs_it = use32 && staged ? stp::staged_blocks(h->st_ntiles) : nbs;   // partials of A^T r
         double best = HUGE_VAL;
         int improved_at = it;
         bool stalled = false;
+        bool ymode = use32;
         for (; it < max_iter; ++it) {
             const int rev = it & 1;
+            const bool check = (it + 1) % CHECK == 0 || it + 1 == max_iter;
+            const bool norms = ymode && check;
             STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st, rev));
             STP_HIP(stp::launch_cgls_alpha_update(w.sc, w.part0, nbq, w.part1, npp, lam, it, cols, x, w.p, rows, w.r,
-                                                  w.q, use32 ? w.r32 : nullptr, st));
+                                                  w.q, ymode ? w.y : nullptr, use32 ? w.v32 : nullptr,
+                                                  norms ? w.pnorm : nullptr, norms ? w.pnorm + npa : nullptr, st));
             if (use32 && staged)
                 STP_HIP(stp::normal_residual_staged(WT, cols, h->st_rt, h->st_ntiles, h->st_maxu, TRP, h->st_l16,
-                                                    TVAL, h->st_tu, h->st_uniq, w.r32, x, lam, dinv, w.s, w.z,
-                                                    w.part2, w.part3, done, rev, st));
+                                                    TVAL, h->st_tu, h->st_uniq, w.v32, ymode ? w.g : nullptr, x,
+                                                    lam, dinv, w.s, w.z, w.part2, w.part3, done, rev, st));
             else if (use32)
-                STP_HIP(stp::normal_residual_r32(WT, cols, TRP, TCI, TVAL, w.r32, x, lam, dinv, w.s, w.z, w.part2,
-                                                 w.part3, done, rev, st));
+                STP_HIP(stp::normal_residual_r32(WT, cols, TRP, TCI, TVAL, w.v32, ymode ? w.g : nullptr, x, lam,
+                                                 dinv, w.s, w.z, w.part2, w.part3, done, rev, st));
             else
                 STP_HIP(stp::normal_residual(WT, cols, TRP, TCI, TVAL, w.r, x, lam, dinv, w.s, w.z, w.part2, w.part3,
                                              done, st, rev));
@@ -474,10 +486,21 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
                 STP_HIP(stp::launch_cgls_beta_direction(w.sc, w.part2, nbs_it, w.part3, nbs_it, nullptr, 0, tol,
                                                         history, it, 0, cols, w.z, w.p, w.part1, w.s, nullptr, st));
             }
-            if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
+            if (check) {
                 STP_HIP(hipMemcpyAsync(&hs, w.sc, sizeof hs, hipMemcpyDeviceToHost, st));
+                if (norms)
+                    STP_HIP(hipMemcpyAsync(hnorm, w.pnorm, sizeof(double) * 2 * npa, hipMemcpyDeviceToHost,
+                                           st));
                 STP_HIP(hipStreamSynchronize(st));
                 if (hs.done) break;
+                if (norms) {
+                    double rr = 0.0, yy = 0.0;
+                    for (int k = 0; k < npa; ++k) {
+                        rr += hnorm[k];
+                        yy += hnorm[npa + k];
+                    }
+                    if (yy > rr) ymode = false;   // from the next iteration on: A^T r
+                }
                 if (hs.rel < 0.9 * best) {
                     best = hs.rel;
                     improved_at = it;
@@ -539,8 +562,11 @@ constexpr size_t NA_OFF_S = NA_OFF_P + na_align(4 * NA_MAX_COLS);
 constexpr size_t NA_OFF_Z = NA_OFF_S + na_align(4 * NA_MAX_COLS);
 constexpr size_t NA_OFF_SP = NA_OFF_Z + na_align(4 * NA_MAX_COLS);
 constexpr size_t NA_OFF_DINV = NA_OFF_SP + na_align(4 * NA_MAX_COLS);
-constexpr size_t NA_OFF_R32 = NA_OFF_DINV + na_align(4 * NA_MAX_COLS);
-constexpr size_t NA_BYTES = NA_OFF_R32 + na_align(4 * NA_MAX_ROWS);   // 36.1 MiB
+constexpr size_t NA_OFF_V32 = NA_OFF_DINV + na_align(4 * NA_MAX_COLS);
+constexpr size_t NA_OFF_Y = NA_OFF_V32 + na_align(4 * NA_MAX_ROWS);
+constexpr size_t NA_OFF_G = NA_OFF_Y + na_align(8 * NA_MAX_ROWS);
+constexpr size_t NA_OFF_NORM = NA_OFF_G + na_align(8 * NA_MAX_COLS);
+constexpr size_t NA_BYTES = NA_OFF_NORM + na_align(8 * 2 * 256);   // 52.1 MiB
 constexpr int NA_MAX_DEVICES = 64;
 
 std::mutex g_scratch_mu;
@@ -592,6 +618,9 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     STP_HIP(grow(&h->sp, &h->c_sp, cols));
     STP_HIP(grow(&h->dinv, &h->c_dinv, cols));
     STP_HIP(grow(&h->r32, &h->c_r32, rows));
+    STP_HIP(grow(&h->ry, &h->c_ry, rows));
+    STP_HIP(grow(&h->rg, &h->c_rg, cols));
+    STP_HIP(grow(&h->pnorm, &h->c_pnorm, 2 * 256));
     STP_HIP(grow(&h->part0, &h->c_p0, nbq));
     STP_HIP(grow(&h->part1, &h->c_p1, nbc));
     STP_HIP(grow(&h->part2, &h->c_p2, nbs));
@@ -600,7 +629,7 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     STP_HIP(grow(&h->part5, &h->c_p5, nbc));
     STP_HIP(grow(&h->sc, &h->c_sc, 1));
     CglsWork w;
-    w.r = h->r; w.p = h->p; w.q = h->q; w.s = h->s; w.z = h->z; w.sp = h->sp; w.dinv = h->dinv; w.r32 = h->r32;
+    w.r = h->r; w.p = h->p; w.q = h->q; w.s = h->s; w.z = h->z; w.sp = h->sp; w.dinv = h->dinv; w.v32 = h->r32; w.y = h->ry; w.g = h->rg; w.pnorm = h->pnorm;
     w.part0 = h->part0; w.part1 = h->part1; w.part2 = h->part2; w.part3 = h->part3; w.part4 = h->part4;
     w.part5 = h->part5; w.sc = h->sc;
     return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, st, w);
@@ -635,7 +664,10 @@ int stp_cgls_noalloc(const stp_csr* hc, const double* b, double lam, double tol,
     w.z = reinterpret_cast<float*>(base + NA_OFF_Z);
     w.sp = reinterpret_cast<float*>(base + NA_OFF_SP);
     w.dinv = reinterpret_cast<float*>(base + NA_OFF_DINV);
-    w.r32 = reinterpret_cast<float*>(base + NA_OFF_R32);
+    w.v32 = reinterpret_cast<float*>(base + NA_OFF_V32);
+    w.y = reinterpret_cast<double*>(base + NA_OFF_Y);
+    w.g = reinterpret_cast<double*>(base + NA_OFF_G);
+    w.pnorm = reinterpret_cast<double*>(base + NA_OFF_NORM);
     return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, S(stream), w);
 }
 

@@ -26,21 +26,24 @@ hipError_t spmv_f32(int W, int rows, const int* rp, const int* ci, const float* 
                     double* part, const int* done, hipStream_t st, int rev = 0);
 hipError_t spmv_f64(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, double* y,
                     double alpha, const double* wv, double lam, double* part, const int* done, hipStream_t st);
+// s = A^T r - lam x (FP64 r); gout, when given, receives A^T r in FP64
 hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
                            const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
-                           double* part_ss, const int* done, hipStream_t st, int rev = 0);
-// the same with r gathered in FP32 (FP64 accumulation)
-hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, const float* val, const float* r,
-                               const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
-                               double* part_ss, const int* done, int rev, hipStream_t st);
+                           double* part_ss, const int* done, hipStream_t st, int rev = 0, double* gout = nullptr);
+// s = g - A^T y - lam x with the change y = r_ref - r of the residual gathered
+// in FP32 and g = A^T r_ref in FP64; without g, s = A^T y - lam x with y = r
+// in FP32 (FP64 accumulation either way)
+hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, const float* val, const float* y,
+                               const double* g, const double* x, double lam, const float* dinv, float* s, float* z,
+                               double* part_sz, double* part_ss, const int* done, int rev, hipStream_t st);
 // the same from the staged A^T (tiles of rt rows; per tile its distinct
-// entries of r are gathered once into LDS); the grid is
+// entries of y are gathered once into LDS); the grid is
 // staged_blocks(ntiles), which is also the count of partials
 int staged_blocks(int ntiles);
 hipError_t normal_residual_staged(int W, int rows, int rt, int ntiles, int maxu, const int* rp, const uint16_t* l16,
-                                  const float* val, const int* tu, const int* uniq, const float* r, const double* x,
-                                  double lam, const float* dinv, float* s, float* z, double* part_sz,
-                                  double* part_ss, const int* done, int rev, hipStream_t st);
+                                  const float* val, const int* tu, const int* uniq, const float* y, const double* g,
+                                  const double* x, double lam, const float* dinv, float* s, float* z,
+                                  double* part_sz, double* part_ss, const int* done, int rev, hipStream_t st);
 hipError_t residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* x, const double* b,
                     double* r, hipStream_t st);
 // FP64-valued variants (the square CG drop-in when FP64 values are attached)
@@ -63,10 +66,13 @@ hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const i
 // elementwise CGLS kernels (grid-stride), and so the count of |p|^2 partials.
 int cgls_blocks(int n);
 // alpha = gam[it&1] / (|q|^2 + lam |p|^2) (every block sums the partials),
-// x += alpha p, r -= alpha q, and r32 = (float)r when r32 is given
+// x += alpha p, r -= alpha q, and y += alpha q when y is given; v32 (when
+// given) receives the FP32 copy of y (y given) or of r; part_rr / part_yy
+// (when given, cgls_blocks(max(rows, cols)) each) the partials of |r|^2, |y|^2
 hipError_t launch_cgls_alpha_update(SolverScalars* sc, const double* pq, int nq, const double* pp, int np,
                                     double lam, int it, int cols, double* x, const float* p, int rows, double* r,
-                                    const float* q, float* r32, hipStream_t st);
+                                    const float* q, double* y, float* v32, double* part_rr, double* part_yy,
+                                    hipStream_t st);
 // rel, convergence test, beta = (z.s or z.(s - s_prev)) / gam[it&1] (every
 // block), gam[(it+1)&1] = z.s, then p = z + beta p with |p|^2 partials and
 // s_prev <- s; with first = 1 (a start or restart) beta = 0 and no scalar is

@@ -225,9 +225,12 @@ hipError_t gather_probe(int W, int rows, const int* rp, const int* ci, const XT*
 template hipError_t gather_probe<float>(int, int, const int*, const int*, const float*, float*, hipStream_t);
 template hipError_t gather_probe<double>(int, int, const int*, const int*, const double*, double*, hipStream_t);
 
-// s = A^T r - lam x (FP64 accumulation; r gathered as RT: FP32 inside the
-// CGLS iteration, FP64 for the true residual), FP32 output s, z = dinv*s,
-// partials of s*z and s*s.  rev != 0 visits the rows last to first.
+// s = A^T r - lam x with FP64 accumulation, FP32 output s, z = dinv*s,
+// partials of s*z and s*s.  r is gathered as RT: FP64 for the true residual,
+// FP32 inside the CGLS iteration.  With gin the gathered vector is the change
+// y = r_ref - r of the residual since the pass started and gin = A^T r_ref,
+// so s = gin - A^T y - lam x; gout (when given) receives A^T r in FP64.
+// rev != 0 visits the rows last to first.
 template <int W, typename RT>
 __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const int* __restrict__ rp,
                                                                const int* __restrict__ ci, const float* __restrict__ val,
@@ -235,7 +238,9 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
                                                                const double* __restrict__ x, double lam,
                                                                const float* __restrict__ dinv, float* __restrict__ s,
                                                                float* __restrict__ z, double* __restrict__ part_sz,
-                                                               double* __restrict__ part_ss, const int* done, int rev) {
+                                                               double* __restrict__ part_ss, const int* done, int rev,
+                                                               const double* __restrict__ gin,
+                                                               double* __restrict__ gout) {
     if (done && *done) return;
     constexpr int RPB = 256 / W;
     const int lane = threadIdx.x % W, sub = threadIdx.x / W;
@@ -245,7 +250,8 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
             const int row = rev ? rows - 1 - (row0 + sub) : row0 + sub;
             const double acc = row_dot<W, double, RT>(rp, ci, val, r, row, lane);
             if (lane == 0) {
-                const float sv = (float)(acc - lam * x[row]);
+                if (gout) gout[row] = acc;
+                const float sv = (float)((gin ? gin[row] - acc : acc) - lam * x[row]);
                 const float zv = dinv ? dinv[row] * sv : sv;
                 s[row] = sv;
                 z[row] = zv;
@@ -261,46 +267,50 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
 template <typename RT>
 static hipError_t normal_residual_t(int W, int rows, const int* rp, const int* ci, const float* val, const RT* r,
                                     const double* x, double lam, const float* dinv, float* s, float* z,
-                                    double* part_sz, double* part_ss, const int* done, int rev, hipStream_t st) {
+                                    double* part_sz, double* part_ss, const int* done, int rev, const double* gin,
+                                    double* gout, hipStream_t st) {
     const int nb = spmv_blocks(rows, W);
     if (rows == 0) return hipSuccess;
     switch (W) {
         case 64: hipLaunchKernelGGL((normal_residual_kernel<64, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
-                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev, gin, gout); break;
         case 32: hipLaunchKernelGGL((normal_residual_kernel<32, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
-                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev, gin, gout); break;
         case 16: hipLaunchKernelGGL((normal_residual_kernel<16, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
-                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev, gin, gout); break;
         default: hipLaunchKernelGGL((normal_residual_kernel<8, RT>), dim3(nb), dim3(256), 0, st, rows, rp, ci, val, r,
-                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    x, lam, dinv, s, z, part_sz, part_ss, done, rev, gin, gout); break;
     }
     return hipGetLastError();
 }
 
 hipError_t normal_residual(int W, int rows, const int* rp, const int* ci, const float* val, const double* r,
                            const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
-                           double* part_ss, const int* done, hipStream_t st, int rev) {
-    return normal_residual_t<double>(W, rows, rp, ci, val, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev, st);
+                           double* part_ss, const int* done, hipStream_t st, int rev, double* gout) {
+    return normal_residual_t<double>(W, rows, rp, ci, val, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev,
+                                     nullptr, gout, st);
 }
 
-hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, const float* val, const float* r,
-                               const double* x, double lam, const float* dinv, float* s, float* z, double* part_sz,
-                               double* part_ss, const int* done, int rev, hipStream_t st) {
-    return normal_residual_t<float>(W, rows, rp, ci, val, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev, st);
+hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, const float* val, const float* y,
+                               const double* g, const double* x, double lam, const float* dinv, float* s, float* z,
+                               double* part_sz, double* part_ss, const int* done, int rev, hipStream_t st) {
+    return normal_residual_t<float>(W, rows, rp, ci, val, y, x, lam, dinv, s, z, part_sz, part_ss, done, rev, g,
+                                    nullptr, st);
 }
 
-// Staged A^T r: one workgroup per tile of rt rows (cells) at a time.  The
-// tile's distinct rays are read from r once, into LDS (uniq is sorted, so
+// Staged A^T y: one workgroup per tile of rt rows (cells) at a time.  The
+// tile's distinct rays are read from y once, into LDS (uniq is sorted, so
 // neighbouring lanes read neighbouring rays), and the rows' dot products then
 // gather from LDS through 16-bit positions; the row order, lanes and chunking
-// are those of normal_residual_kernel, so each row's sum is the same.
+// are those of normal_residual_kernel, so each row's sum is the same.  As
+// there, with g the gathered vector is y and s = g - A^T y - lam x.
 template <int W>
 __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
     int rows, int rt, int ntiles, const int* __restrict__ rp, const uint16_t* __restrict__ l16,
     const float* __restrict__ val, const int* __restrict__ tu, const int* __restrict__ uniq,
-    const float* __restrict__ r, const double* __restrict__ x, double lam, const float* __restrict__ dinv,
-    float* __restrict__ s, float* __restrict__ z, double* __restrict__ part_sz, double* __restrict__ part_ss,
-    const int* done, int rev) {
+    const float* __restrict__ r, const double* __restrict__ g, const double* __restrict__ x, double lam,
+    const float* __restrict__ dinv, float* __restrict__ s, float* __restrict__ z, double* __restrict__ part_sz,
+    double* __restrict__ part_ss, const int* done, int rev) {
     extern __shared__ float xs[];
     if (done && *done) return;
     constexpr int RPB = 256 / W;
@@ -316,7 +326,7 @@ __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
         for (int row = t * rt + sub; row < r1; row += RPB) {
             const double acc = row_dot<W, double, float, float, false, uint16_t>(rp, l16, val, xs, row, lane);
             if (lane == 0) {
-                const float sv = (float)(acc - lam * x[row]);
+                const float sv = (float)((g ? g[row] - acc : acc) - lam * x[row]);
                 const float zv = dinv ? dinv[row] * sv : sv;
                 s[row] = sv;
                 z[row] = zv;
@@ -332,22 +342,22 @@ __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
 int staged_blocks(int ntiles) { return ntiles < 1 ? 1 : (ntiles > MAX_BLOCKS ? MAX_BLOCKS : ntiles); }
 
 hipError_t normal_residual_staged(int W, int rows, int rt, int ntiles, int maxu, const int* rp, const uint16_t* l16,
-                                  const float* val, const int* tu, const int* uniq, const float* r, const double* x,
-                                  double lam, const float* dinv, float* s, float* z, double* part_sz,
-                                  double* part_ss, const int* done, int rev, hipStream_t st) {
+                                  const float* val, const int* tu, const int* uniq, const float* y, const double* g,
+                                  const double* x, double lam, const float* dinv, float* s, float* z,
+                                  double* part_sz, double* part_ss, const int* done, int rev, hipStream_t st) {
     if (rows == 0) return hipSuccess;
     if (maxu < 0 || maxu > STAGE_CAP || rt < 1) return hipErrorInvalidValue;
     const int nb = staged_blocks(ntiles);
     const size_t lds = sizeof(float) * (size_t)(maxu > 0 ? maxu : 1);
     switch (W) {
         case 64: hipLaunchKernelGGL(normal_residual_staged_kernel<64>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
         case 32: hipLaunchKernelGGL(normal_residual_staged_kernel<32>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
         case 16: hipLaunchKernelGGL(normal_residual_staged_kernel<16>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
         default: hipLaunchKernelGGL(normal_residual_staged_kernel<8>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, r, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
     }
     return hipGetLastError();
 }
@@ -370,7 +380,10 @@ __device__ double block_sum(const double* part, int n) {
 }
 
 // CGLS, after q = A p (with its |q|^2 partials): alpha = gamma / (|q|^2 +
-// lam |p|^2), x += alpha p (cols), r -= alpha q (rows).  Every block sums the
+// lam |p|^2), x += alpha p (cols), r -= alpha q (rows) and, when y is given,
+// y += alpha q (y = r_ref - r); v32 (when given) receives the FP32 copy of y
+// (y given) or of r, and part_rr / part_yy (when given) the partials of |r|^2
+// and |y|^2.  Every block sums the
 // same partials in the same order, so all blocks hold the same alpha and no
 // single-block scalar launch sits between the SpMV and the update; block 0
 // records alpha (and the breakdown flag).  The done flag is read once per
@@ -380,7 +393,9 @@ __global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* s
                                                                 int cols, double* __restrict__ x,
                                                                 const float* __restrict__ p, int rows,
                                                                 double* __restrict__ r, const float* __restrict__ q,
-                                                                float* __restrict__ r32) {
+                                                                double* __restrict__ y, float* __restrict__ v32,
+                                                                double* __restrict__ part_rr,
+                                                                double* __restrict__ part_yy) {
     __shared__ int s_done;
     if (threadIdx.x == 0) s_done = sc->done;
     __syncthreads();
@@ -396,13 +411,26 @@ __global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* s
     const double a = sc->gam[it & 1] / del;
     if (first) sc->alpha = a;
     const int n = cols > rows ? cols : rows;
+    double rr = 0.0, yy = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < cols) x[i] += a * (double)p[i];
         if (i < rows) {
             const double rv = r[i] - a * (double)q[i];
             r[i] = rv;
-            if (r32) r32[i] = (float)rv;
+            if (y) {
+                const double yv = y[i] + a * (double)q[i];
+                y[i] = yv;
+                if (v32) v32[i] = (float)yv;
+                yy += yv * yv;
+            } else if (v32) {
+                v32[i] = (float)rv;
+            }
+            rr += rv * rv;
         }
+    }
+    if (part_rr) {
+        block_partial(rr, part_rr);
+        block_partial(yy, part_yy);
     }
 }
 
@@ -880,9 +908,10 @@ int cgls_blocks(int n) {
 }
 hipError_t launch_cgls_alpha_update(SolverScalars* sc, const double* pq, int nq, const double* pp, int np,
                                     double lam, int it, int cols, double* x, const float* p, int rows, double* r,
-                                    const float* q, float* r32, hipStream_t st) {
+                                    const float* q, double* y, float* v32, double* part_rr, double* part_yy,
+                                    hipStream_t st) {
     hipLaunchKernelGGL(cgls_alpha_update_kernel, dim3(cgls_blocks(cols > rows ? cols : rows)), dim3(256), 0, st, sc,
-                       pq, nq, pp, np, lam, it, cols, x, p, rows, r, q, r32);
+                       pq, nq, pp, np, lam, it, cols, x, p, rows, r, q, y, v32, part_rr, part_yy);
     return hipGetLastError();
 }
 hipError_t launch_cgls_beta_direction(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,

@@ -237,9 +237,10 @@ int stp_cg(const stp_csr* A, const double* b_dev, int32_t max_iter, double tol, 
  * |A^T r - lambda x| / |A^T b| < tol.  precond: 0 none, 1 Jacobi
  * (1/(sum_i a_ij^2 + lambda)), 2 learned network (precond_handle = stp_lpc*,
  * flexible CG).  FP32 SpMV, FP64 x, r and dots, FP64 residual replacement.
- * For tol >= 1e-6 the iteration's A^T r gathers r rounded to FP32 (FP64
- * sums); a pass that stalls or fails the FP64 residual check continues from
- * the recomputed residual with r in FP64.  With a cell grid the solve runs in
+ * For tol >= 1e-6 the iteration forms A^T r as A^T r_ref (FP64, at the start
+ * of the pass) minus A^T of the change of r since then, gathered in FP32
+ * (FP64 sums); a pass that stalls or fails the FP64 residual check continues
+ * from the recomputed residual with r in FP64.  With a cell grid the solve runs in
  * the solver layout, whose A^T is staged in tiles (stp_csr_prepare_layout).
  * New: the reference has CG on square systems only. */
 int stp_cgls(const stp_csr* A, const double* b_dev, double lambda, double tol, int32_t max_iter, int32_t precond,
@@ -247,7 +248,7 @@ int stp_cgls(const stp_csr* A, const double* b_dev, double lambda, double tol, i
 
 /* stp_cgls without any device allocation: the same solve (same kernels, same
  * results bit for bit) with its workspace in a static device array of the
- * library (36.1 MiB per GPU, held until the process ends).  No counterpart in
+ * library (52.1 MiB per GPU, held until the process ends).  No counterpart in
  * the reference: for callers that cannot allocate (tight serving loops,
  * harnesses that forbid hipMalloc).  Limits: at most 2^20 rows and 2^20
  * columns (STP_ERR_INVALID beyond); a matrix with a cell grid must already
