@@ -65,15 +65,23 @@ __device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const IT* __re
     return acc;
 }
 
+// Block reductions for 256-thread blocks (4 wavefronts): a butterfly of
+// cross-lane adds inside each wavefront (every lane ends with the same sum),
+// then the four wavefront sums added in wavefront order.  The order is fixed,
+// so the sums are reproducible.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
 __device__ __forceinline__ void block_partial(double v, double* part) {
-    __shared__ double red[256];
-    red[threadIdx.x] = v;
+    __shared__ double red[4];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    __syncthreads();   // red is reused by the next call
 }
 
 template <int W, typename XT, typename AT, typename YT, bool FUSE_W, bool PART, typename VT = float>
@@ -364,18 +372,36 @@ hipError_t normal_residual_staged(int W, int rows, int rt, int ntiles, int maxu,
 
 // ------------------------------------------------------------------ reductions / scalars
 // Sum `n` partials into *out with one 256-thread block (fixed order: deterministic).
-__device__ double block_sum(const double* part, int n) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+// Sums of up to three partial arrays (b, c optional), every thread of the
+// block receives them; each thread adds its strided share in index order.
+__device__ void block_sums(const double* a, int na, const double* b, int nb, const double* c, int nc, double* ra,
+                           double* rb, double* rc) {
+    __shared__ double red[3][4];
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+    for (int i = threadIdx.x; i < na; i += 256) sa += a[i];
+    if (b)
+        for (int i = threadIdx.x; i < nb; i += 256) sb += b[i];
+    if (c)
+        for (int i = threadIdx.x; i < nc; i += 256) sc += c[i];
+    sa = wave_sum(sa);
+    sb = wave_sum(sb);
+    sc = wave_sum(sc);
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        red[0][w] = sa;
+        red[1][w] = sb;
+        red[2][w] = sc;
     }
-    const double r = red[0];
     __syncthreads();
+    *ra = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    if (rb) *rb = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
+    if (rc) *rc = ((red[2][0] + red[2][1]) + red[2][2]) + red[2][3];
+    __syncthreads();   // red is reused by the next call
+}
+
+__device__ double block_sum(const double* part, int n) {
+    double r;
+    block_sums(part, n, nullptr, 0, nullptr, 0, &r, nullptr, nullptr);
     return r;
 }
 
@@ -400,8 +426,8 @@ __global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* s
     if (threadIdx.x == 0) s_done = sc->done;
     __syncthreads();
     if (s_done) return;
-    const double qq = block_sum(pq, nq);
-    const double ppv = block_sum(pp, np);
+    double qq, ppv;
+    block_sums(pq, nq, pp, np, nullptr, 0, &qq, &ppv, nullptr);
     const double del = qq + lam * ppv;
     const bool first = blockIdx.x == 0 && threadIdx.x == 0;
     if (!(del > 0.0)) {
@@ -455,9 +481,8 @@ __global__ void __launch_bounds__(256) cgls_beta_direction_kernel(SolverScalars*
     if (s_done) return;
     double b = 0.0;
     if constexpr (!FIRST) {
-        const double sz = block_sum(psz, nsz);
-        const double ss = block_sum(pss, nss);
-        const double fl = pflex ? block_sum(pflex, nfl) : 0.0;
+        double sz, ss, fl = 0.0;
+        block_sums(psz, nsz, pss, nss, pflex, nfl, &sz, &ss, pflex ? &fl : nullptr);
         const double rel = sc->s0 > 0.0 ? sqrt(ss) / sc->s0 : 0.0;
         const bool first = blockIdx.x == 0 && threadIdx.x == 0;
         if (first) {
@@ -482,14 +507,7 @@ __global__ void __launch_bounds__(256) cgls_beta_direction_kernel(SolverScalars*
         v += (double)pv * (double)pv;
         if (s_prev) s_prev[i] = s[i];
     }
-    __shared__ double red[256];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part_pp[blockIdx.x] = red[0];
+    block_partial(v, part_pp);
 }
 
 // partials of z.s and z.(s - s_prev) (flexible variant)
@@ -502,18 +520,8 @@ __global__ void flex_dots_kernel(const int* done, int n, const float* z, const f
         a = (double)z[i] * (double)s[i];
         b = (double)z[i] * ((double)s[i] - (double)s_prev[i]);
     }
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = a;
-    red[1][threadIdx.x] = b;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + o];
-            red[1][threadIdx.x] += red[1][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { part_zs[blockIdx.x] = red[0][0]; part_fl[blockIdx.x] = red[1][0]; }
+    block_partial(a, part_zs);
+    block_partial(b, part_fl);
 }
 
 // gamma <- sum(psz), s0 <- sqrt(sum(pss)) (initialisation) or only gamma
@@ -521,8 +529,8 @@ __global__ void flex_dots_kernel(const int* done, int n, const float* z, const f
 // cleared here, so the direction kernel that follows runs)
 __global__ void cgls_init_kernel(SolverScalars* sc, const double* psz, int nsz, const double* pss, int nss,
                                  int set_s0, int it) {
-    const double sz = block_sum(psz, nsz);
-    const double ss = block_sum(pss, nss);
+    double sz, ss;
+    block_sums(psz, nsz, pss, nss, nullptr, 0, &sz, &ss, nullptr);
     if (threadIdx.x == 0) {
         sc->gamma = sz;
         sc->gam[it & 1] = sz;
@@ -624,14 +632,7 @@ __global__ void dot_partial_kernel(const int* done, int n, const double* a, cons
     if (done && *done) return;
     double v = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += a[i] * b[i];
-    __shared__ double red[256];
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+    block_partial(v, part);
 }
 
 __global__ void cg_alpha_kernel(SolverScalars* sc, const double* part, int n) {
@@ -658,26 +659,16 @@ __global__ void cg_update_kernel(const SolverScalars* sc, int n, double* x, doub
         rr += ri * ri;
         rz += ri * zi;
     }
-    __shared__ double red[2][256];
-    red[0][threadIdx.x] = rr;
-    red[1][threadIdx.x] = rz;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + o];
-            red[1][threadIdx.x] += red[1][threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) { part_rr[blockIdx.x] = red[0][0]; part_rz[blockIdx.x] = red[1][0]; }
+    block_partial(rr, part_rr);
+    block_partial(rz, part_rz);
 }
 
 // convergence test on |r|/|b| (reference: CG and PCG both test before the p update)
 __global__ void cg_beta_kernel(SolverScalars* sc, const double* prr, const double* prz, int n, double tol, int it,
                                double* history) {
     if (sc->done) return;
-    const double rr = block_sum(prr, n);
-    const double rz = block_sum(prz, n);
+    double rr, rz;
+    block_sums(prr, n, prz, n, nullptr, 0, &rr, &rz, nullptr);
     if (threadIdx.x == 0) {
         sc->iters = it + 1;
         sc->rel = sqrt(rr) / sc->s0;
@@ -695,8 +686,8 @@ __global__ void cg_direction_kernel(const SolverScalars* sc, int n, const double
 }
 
 __global__ void cg_init_kernel(SolverScalars* sc, const double* prr, const double* prz, int n) {
-    const double rr = block_sum(prr, n);
-    const double rz = block_sum(prz, n);
+    double rr, rz;
+    block_sums(prr, n, prz, n, nullptr, 0, &rr, &rz, nullptr);
     if (threadIdx.x == 0) {
         sc->s0 = sqrt(rr);
         sc->gamma = rz;
