@@ -169,9 +169,12 @@ hipError_t sort_pairs(K* k0, K* k1, int* v0, int* v1, int n, int end_bit, hipStr
 }
 
 // Staged copy of the layout A^T (n rows, entries reading a vector of length
-// g): the largest tile height in {256, 128, 64, 32} rows whose tiles each
-// read at most STAGE_CAP distinct entries of that vector.  None fitting
-// leaves st_rt = 0 (the CGLS iteration then gathers from global memory).
+// g): tiles of 64 rows, or 32 when a 64-row tile would read more than
+// STAGE_CAP distinct entries of that vector.  Taller tiles reuse more (at
+// 200k rays / 64^3 each staged ray serves 4.0 entries at 64 rows, 4.8 at
+// 128) but give half as many workgroups to balance: CGLS took 34.9 ms at 64
+// rows, 36.7 at 128, 42.9 at 256.  None fitting leaves st_rt = 0 (the CGLS
+// iteration then gathers from global memory).
 hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
     const int n = h->cols;
     const int nnz = (int)h->nnz;
@@ -186,7 +189,7 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
         if ((e = hipMalloc(&i1, sizeof(int) * nnz)) != hipSuccess) break;
         if ((e = hipMalloc(&flag, sizeof(int) * ((size_t)nnz + 1))) != hipSuccess) break;
         if ((e = hipMalloc(&uidx, sizeof(int) * ((size_t)nnz + 1))) != hipSuccess) break;
-        for (int rt = 256; rt >= 32; rt /= 2) {
+        for (int rt = 64; rt >= 32; rt /= 2) {
             const int nt = (n + rt - 1) / rt;
             hipFree(tu);
             tu = nullptr;
