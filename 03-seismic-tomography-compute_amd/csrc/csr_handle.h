@@ -38,6 +38,7 @@ struct stp_csr {
     // column in that list (st_l16); st_rt = 0 when no tile size fits the
     // staging buffer
     int st_rt = 0, st_ntiles = 0, st_maxu = 0;
+    int64_t st_nlist = 0;   // total length of the tiles' lists
     int *st_tu = nullptr, *st_uniq = nullptr;
     uint16_t* st_l16 = nullptr;
     size_t c_dp = 0, c_dap = 0, c_dz = 0, c_dr = 0, c_ddinv = 0, c_sc = 0;

@@ -222,6 +222,7 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
             h->st_rt = rt;
             h->st_ntiles = nt;
             h->st_maxu = maxu;
+            h->st_nlist = htu[(size_t)nt];
             break;
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -243,6 +244,7 @@ void free_solver_layout(stp_csr* h) {
     h->st_tu = h->st_uniq = nullptr;
     h->st_l16 = nullptr;
     h->st_rt = h->st_ntiles = h->st_maxu = 0;
+    h->st_nlist = 0;
     h->lay_ready = false;
 }
 
@@ -356,6 +358,13 @@ int stp_csr_set_grid(stp_csr* h, int64_t nx, int64_t ny, int64_t nz) {
 int stp_csr_prepare_layout(stp_csr* h, void* stream) {
     if (!h) return stp::fail(STP_ERR_INVALID, "stp_csr_prepare_layout: null handle");
     return stp::build_solver_layout(h, stp::S(stream));
+}
+
+int stp_csr_staging(const stp_csr* h, int32_t* tile_rows, int64_t* list_entries) {
+    if (!h || !tile_rows || !list_entries) return stp::fail(STP_ERR_INVALID, "stp_csr_staging: null argument");
+    *tile_rows = h->lay_ready ? h->st_rt : -1;
+    *list_entries = h->lay_ready ? h->st_nlist : 0;
+    return STP_OK;
 }
 
 }  // extern "C"

@@ -76,6 +76,7 @@ SIGNATURES: dict[str, list] = {
     "stp_csr_spmv_f64": [_vp, _i32, _vp, _vp, _vp],
     "stp_csr_set_grid": [_vp, _i64, _i64, _i64],
     "stp_csr_prepare_layout": [_vp, _vp],
+    "stp_csr_staging": [_vp, C.POINTER(_i32), C.POINTER(_i64)],
     "stp_csr_set_values_f64": [_vp, _vp, _vp],
     "stp_csr_gather_probe": [_vp, _i32, _i32, _vp, _vp, _vp],
     "stp_cg": [_vp, _vp, _i32, _f64, _i32, _vp, _vp, C.POINTER(SolveInfo), _vp],

@@ -86,6 +86,14 @@ class DeviceCSR:
         """Build the solver layout now instead of on the first CGLS solve."""
         call("stp_csr_prepare_layout", self._h, stream_handle())
 
+    def staging(self) -> tuple[int, int]:
+        """(rows per tile of the staged A^T, length of the tiles' lists of
+        distinct rows); rows per tile 0 when A^T is not staged, -1 before the
+        solver layout is built (stp_csr_staging)."""
+        t, n = C.c_int32(), C.c_int64()
+        call("stp_csr_staging", self._h, C.byref(t), C.byref(n))
+        return t.value, n.value
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):

@@ -692,7 +692,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=64, help="time steps of the CPU baseline sample")
     ap.add_argument("--no-cg", action="store_true", help="skip the tomography-solve timing")
     ap.add_argument("--no-rtm", action="store_true", help="skip the RTM imaging timing")
     ap.add_argument("--sponge", default="cerjan", choices=("cerjan", "reference"),
@@ -855,7 +855,7 @@ def main() -> int:
         steps = args.cpu_steps
         log(f"cpu baseline: FP64 restatement on the {n}^3 workload, first {steps} steps, {args.cpu_threads} threads")
         gcs, sec = cpu_baseline(args.cpu_threads, vel_h, n, steps, args.sponge, rcv=rcv)
-        steps1 = 2
+        steps1 = 4
         log(f"cpu baseline: the same on 1 core, first {steps1} steps")
         gcs1, sec1 = cpu_baseline(1, vel_h, n, steps1, args.sponge, rcv=rcv)
         omp_threads(args.cpu_threads)

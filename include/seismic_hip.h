@@ -206,6 +206,12 @@ int stp_csr_spmv_f64(const stp_csr* A, int32_t transpose, const double* x_dev, d
 int stp_csr_set_grid(stp_csr* A, int64_t nx, int64_t ny, int64_t nz);
 /* Build that layout now (no-op without a grid or when built).  Synchronises. */
 int stp_csr_prepare_layout(stp_csr* A, void* stream);
+/* The staged form of the layout's A^T that the CGLS iteration reads: rows per
+ * tile (64 or 32; 0 when no tile height keeps a tile's distinct rows within
+ * the staging buffer, and the iteration gathers from global memory; -1 when
+ * the layout is not built) and the total length of the tiles' lists of
+ * distinct rows.  New: the reference has no such layout. */
+int stp_csr_staging(const stp_csr* A, int32_t* tile_rows_host, int64_t* list_entries_host);
 
 /* Measurement, no counterpart in the reference: the gathers and the index
  * stream of one product y = A x (transpose = 0) or A^T x (1) with FP32

@@ -108,3 +108,39 @@ def test_products_through_the_layout_match_the_given_order(n, cuda):
         p, q = p.double().cpu().numpy(), q.double().cpu().numpy()
         tol = 1e-13 if k < 2 else 1e-6          # FP64 / FP32 accumulation, different summation order
         assert rel(q, p) < tol, (k, rel(q, p))
+
+
+@pytest.mark.parametrize("cells_per_row,rows,tile", [(0, 0, 64), (5, 20_000, 32), (8, 30_000, 0)])
+def test_staged_transpose_tile_heights(cells_per_row, rows, tile, cuda):
+    """The staged A^T takes 64-cell tiles; 32-cell tiles when a 64-cell tile
+    would read more than 8192 distinct rows; none (global gathers) when even a
+    32-cell tile would.  Random rows over an 8^3 grid reach the last two
+    cases; every case solves to the dense FP64 normal-equation solution."""
+    rng = np.random.default_rng(71)
+    if cells_per_row == 0:
+        n = (12, 12, 12)
+        A = DeviceCSR.from_rays(synthetic.ray_endpoints(3_000, n, seed=72), n)
+        rp_, ci_, v_ = (t.cpu().numpy() for t in A.export())
+    else:
+        n = (8, 8, 8)
+        V = int(np.prod(n))
+        ci_ = np.concatenate([np.sort(rng.choice(V, cells_per_row, replace=False)) for _ in range(rows)])
+        rp_ = np.arange(0, rows * cells_per_row + 1, cells_per_row)
+        v_ = rng.uniform(0.1, 1.0, ci_.size).astype(np.float32)
+        A = DeviceCSR.from_arrays(rp_.astype(np.int32), ci_.astype(np.int32), v_)
+        assert A.cols == V
+        A.set_grid(n)
+    A.prepare_layout()
+    t, nlist = A.staging()
+    assert t == tile
+    assert (nlist > 0) == (tile > 0)
+    V = A.cols
+    dense = np.zeros((A.rows, V))
+    rp_ = rp_.astype(np.int64)
+    for r in range(A.rows):
+        dense[r, ci_[rp_[r]:rp_[r + 1]]] = v_[rp_[r]:rp_[r + 1]]
+    b = rng.standard_normal(A.rows)
+    x, info, _ = A.cgls(torch.as_tensor(b, device="cuda"), LAM, 1e-6, 5000, "jacobi")
+    assert info.converged and info.true_rel_residual < 1e-6
+    xd = np.linalg.solve(dense.T @ dense + LAM * np.eye(V), dense.T @ b)
+    assert rel(x.cpu(), xd) < 1e-4
