@@ -31,20 +31,44 @@ unsigned blocks_for(int64_t n) {
     return (unsigned)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
 }
 
-__global__ void tikhonov_kernel(int64_t nx, int64_t ny, int64_t nz, double lam, const double* __restrict__ v,
-                                double* __restrict__ g) {
+// A thread walks the rows of its block's chunk with the previous, current
+// and next values of its z column in registers (the y neighbours) and takes
+// its z neighbours from the neighbouring lanes; only the first and last lane
+// of a wavefront load theirs.  Per cell: the next row's value, the two x
+// neighbours and g.  Lanes past nz take part in the exchanges and store
+// nothing.
+__global__ void __launch_bounds__(256) tikhonov_kernel(int64_t nx, int64_t ny, int64_t nz, double lam,
+                                                        const double* __restrict__ v, double* __restrict__ g) {
     const int64_t iz = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (iz >= nz) return;
-    const bool zin = iz > 0 && iz < nz - 1;
-    STP_FOR_PLANES_ROWS(ix, iy, nx, ny, stp::ROWS) {
-        const int64_t i = (ix * ny + iy) * nz + iz;
-        const double c = v[i];
-        double lap = 0.0;
-        // same association as the reference: per axis (v+ - 2v) + v-, axes added x, y, z
-        if (ix > 0 && ix < nx - 1) lap += v[i + ny * nz] - 2 * c + v[i - ny * nz];
-        if (iy > 0 && iy < ny - 1) lap += v[i + nz] - 2 * c + v[i - nz];
-        if (zin) lap += v[i + 1] - 2 * c + v[i - 1];
-        g[i] = g[i] - lam * lap;
+    const bool act = iz < nz;
+    const int64_t zc = act ? iz : nz - 1;   // a valid column for the lanes past nz
+    const bool zin = act && iz > 0 && iz < nz - 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t plane = ny * nz;
+    for (int64_t ix = blockIdx.z; ix < nx; ix += gridDim.z) {
+        const bool xin = ix > 0 && ix < nx - 1;
+        for (int64_t y0 = (int64_t)blockIdx.y * stp::ROWS; y0 < ny; y0 += (int64_t)gridDim.y * stp::ROWS) {
+            const int64_t y1 = y0 + stp::ROWS < ny ? y0 + stp::ROWS : ny;
+            const int64_t base = ix * plane + zc;
+            double vm = y0 > 0 ? v[base + (y0 - 1) * nz] : 0.0;
+            double vc = v[base + y0 * nz];
+            for (int64_t iy = y0; iy < y1; ++iy) {
+                const int64_t i = base + iy * nz;
+                const double vp = iy + 1 < ny ? v[i + nz] : 0.0;
+                double zm = __shfl_up(vc, 1), zp = __shfl_down(vc, 1);
+                if (lane == 0 && zin) zm = v[i - 1];
+                if ((lane == 63 || iz + 1 >= nz) && zin) zp = v[i + 1];
+                const double xm = xin ? v[i - plane] : 0.0, xp = xin ? v[i + plane] : 0.0;
+                double lap = 0.0;
+                // same association as the reference: per axis (v+ - 2v) + v-, axes added x, y, z
+                if (xin) lap += xp - 2 * vc + xm;
+                if (iy > 0 && iy < ny - 1) lap += vp - 2 * vc + vm;
+                if (zin) lap += zp - 2 * vc + zm;
+                if (act) g[i] = g[i] - lam * lap;
+                vm = vc;
+                vc = vp;
+            }
+        }
     }
 }
 

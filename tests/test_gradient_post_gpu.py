@@ -23,6 +23,13 @@ def test_post_processing_kernels_match_oracle(cuda):
     gc = GradientComputer(nx, ny, nz, smoothing_sigma=1.5, regularization_lambda=3e-3)
     r = gc.apply_regularization(v, g).cpu().numpy()
     assert np.array_equal(r, ogp.tikhonov(v, g, nx, ny, nz, 3e-3))
+    # z columns across wavefront (64) and workgroup (256) edges, y rows across
+    # the 16-row chunks, and single-cell axes
+    for sx, sy, sz in ((4, 37, 300), (3, 1, 130), (1, 5, 64), (2, 18, 1)):
+        vv = rng.uniform(1500, 4500, sx * sy * sz)
+        gg = rng.standard_normal(sx * sy * sz)
+        rr = GradientComputer(sx, sy, sz, regularization_lambda=0.7).apply_regularization(vv, gg).cpu().numpy()
+        assert np.array_equal(rr, ogp.tikhonov(vv, gg, sx, sy, sz, 0.7)), (sx, sy, sz)
     assert np.array_equal(gc.normalize_gradient(r).cpu().numpy(), ogp.normalize(r))
     tiny = np.full(50, 1e-31)
     assert np.array_equal(gc.normalize_gradient(tiny).cpu().numpy(), tiny)
