@@ -11,6 +11,8 @@ from oracle import fd as ofd
 from oracle import sparse as osp
 from seismic_engine import synthetic
 
+import dropin_sparse_case
+
 pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parent.parent
 DRIVER = ROOT / "03-seismic-tomography-compute_amd" / "lib" / "dropin_driver"
@@ -81,51 +83,10 @@ def test_solver_forward_step_adjoint_match_oracle(tmp_path, cuda):
                          "shown, so the case stays off the GPU (the same C ABI calls pass through Python in "
                          "test_sparse_gpu.py)")
 def test_csr_sparse_ops_match_oracle(tmp_path, cuda):
-    rng = np.random.default_rng(3)
-    rows, cols, m = 40, 30, 900
-    r = rng.integers(-2, rows + 2, m)
-    c = rng.integers(-2, cols + 2, m)
-    v = rng.standard_normal(m)
-    r.astype(np.int64).tofile(tmp_path / "r.i64")
-    c.astype(np.int64).tofile(tmp_path / "c.i64")
-    v.tofile(tmp_path / "v.f64")
-    x = rng.standard_normal(cols)
-    u = rng.standard_normal(rows)
-    x.tofile(tmp_path / "x.f64")
-    u.tofile(tmp_path / "u.f64")
-    fdn = 6
-    fdvel = rng.uniform(1500, 2500, fdn ** 3)
-    fdvel.tofile(tmp_path / "fdvel.f64")
-    text = run("sparse", tmp_path, rows=rows, cols=cols, fdn=fdn)
-    rp_o, ci_o, val_o = osp.csr_from_triplets(rows, cols, r, c, v)
-    assert np.array_equal(np.fromfile(tmp_path / "rp.i64", np.int64), rp_o)
-    assert np.array_equal(np.fromfile(tmp_path / "ci.i64", np.int64), ci_o)
-    assert np.array_equal(f64(tmp_path, "val.f64"), val_o)
-    dense = np.zeros((rows, cols))
-    for i in range(rows):
-        dense[i, ci_o[rp_o[i]:rp_o[i + 1]]] = val_o[rp_o[i]:rp_o[i + 1]]
-    # FP32 matrix values on the device, FP64 vectors and accumulation
-    assert rel(f64(tmp_path, "y.f64"), dense @ x) < 1e-6
-    assert rel(f64(tmp_path, "yt.f64"), dense.T @ u) < 1e-6
-    assert rel(f64(tmp_path, "dd.f64"), (dense @ x)[: rows - 5]) < 1e-6
-    # FD matrix vs the oracle's assembly of the same triplets, CG and PCG vs a dense solve
-    frp = np.fromfile(tmp_path / "fd_rp.i64", np.int64)
-    fci = np.fromfile(tmp_path / "fd_ci.i64", np.int64)
-    fval = f64(tmp_path, "fd_val.f64")
-    N = fdn ** 3
-    F = np.zeros((N, N))
-    for i in range(N):
-        F[i, fci[frp[i]:frp[i + 1]]] = fval[frp[i]:frp[i + 1]]
-    assert np.allclose(np.diag(F), 2 - 2 * fdvel ** 2 * 1e-6 * 3 / 100, rtol=1e-12)
-    H = 6 * np.eye(N) - F
-    assert np.allclose(f64(tmp_path, "h_val.f64"), -fval + 6 * (fci == np.repeat(np.arange(N), np.diff(frp))))
-    xs = np.linalg.solve(H, np.ones(N))
-    assert "cg iters" in text and "converged 1 pcg" in text
-    assert rel(f64(tmp_path, "cg.f64"), xs) < 1e-6
-    assert rel(f64(tmp_path, "pcg.f64"), xs) < 1e-6
-    # CGLS on the rectangular matrix vs the damped normal equations
-    xl = np.linalg.solve(dense.T @ dense + 1e-2 * np.eye(cols), dense.T @ u)
-    assert rel(f64(tmp_path, "cgls.f64"), xl) < 1e-5
+    inp = dropin_sparse_case.write_inputs(tmp_path)
+    text = run("sparse", tmp_path, rows=dropin_sparse_case.ROWS, cols=dropin_sparse_case.COLS,
+               fdn=dropin_sparse_case.FDN)
+    dropin_sparse_case.check_outputs(tmp_path, text, inp)
 
 
 def test_rtm_engine_and_run_rtm(tmp_path, cuda):
