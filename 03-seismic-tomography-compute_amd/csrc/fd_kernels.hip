@@ -18,9 +18,15 @@
 //     cells (x-derivative taps come from registers);
 //   * the centre plane x is staged in LDS with an R-cell halo in y and z
 //     (y- and z-derivative taps come from LDS); corners are never loaded
-//     (star stencil); LDS is double buffered, one barrier per plane;
-//   * the loads for plane x+1 (halo, queue front x+1+R, P, C) are issued
-//     before the barrier of plane x so they overlap its arithmetic;
+//     (star stencil); one barrier per plane;
+//   * loads run PD planes ahead (queue front x+PD+R, P and C of x+PD) in
+//     register rings whose slots are compile-time constants (the x loop is
+//     unrolled by the ring length), so each plane waits only for its own
+//     loads; the halo cells of a plane are fetched with its queue front,
+//     R planes before it is the centre, when the neighbouring tiles fetch
+//     the same cells as their own queue front, so they come from L2 (cut
+//     the step's DRAM reads from 1.18x to 1.05x the algorithmic bytes), and
+//     wait in that plane's LDS buffer (R+2 buffers);
 //   * workgroup ids are remapped so each XCD gets a contiguous range of tiles
 //     (neighbouring tiles share that XCD's L2 for their halos).
 // Sponge: D = min(Dx(plane), Dyz(cell)) because the reference factor is a
@@ -34,9 +40,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "fd_kernels.h"
 
 namespace stp {
+
+// non-temporal access flags of the step (template parameter NTL)
+enum { FD_NT_LOADS = 1, FD_NT_STORE = 2 };
 
 template <int VW> struct Vec;
 typedef float f4n __attribute__((ext_vector_type(4)));
@@ -83,6 +95,16 @@ __device__ __forceinline__ int sponge_dist(int i, int nh, int hw, int sw) {
     return d;
 }
 
+// One axis' pair of taps at distance k, in neighbour-minus-centre form: its
+// rounding scales with the differences rather than with |u|.  (The plain
+// pair sum for k >= 2, with their -2u share added once per cell, saves two
+// VALU ops per pair but raised the error of a 2000-step 128^3 run against
+// FP64 from 6e-6 to 1.3e-4.)  The fast and the guard path share it, so a
+// cell's result does not depend on the path that computed it.
+__device__ __forceinline__ float lap_term(float c, float p, float m, float u, float acc) {
+    return fmaf(c, (p - u) + (m - u), acc);
+}
+
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
     // hardware dispatches workgroup b to XCD (b % 8); give each XCD a
     // contiguous range of logical tiles.
@@ -90,7 +112,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
-template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL, int MODE>
+__host__ __device__ constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>), unrolled
+// by construction (register-array indices derived from I stay constant).
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int R, int VW, int TZ, int TY, int TYR, int LX, int NTL, int PD, bool EH, int MODE>
 __global__ void __launch_bounds__((TZ / VW) * (TY / TYR))
 fd_step_kernel(const FdStepArgs a) {
     using V = Vec<VW>;
@@ -104,7 +139,13 @@ fd_step_kernel(const FdStepArgs a) {
     constexpr int NHALO = 2 * R * NTZ + 2 * HV * TY;
     constexpr int NH = (NHALO + NT - 1) / NT; // halo items per thread
 
-    __shared__ vec lds[2][LH][LW];
+    // ring lengths: NS register slots for P, C and halos (loads run PD
+    // planes ahead), NSL LDS plane buffers, UF planes per unrolled pass
+    constexpr int NS = PD + 1;
+    constexpr int NSL = EH ? R + 2 : 2;
+    constexpr int UM = NS / gcd_c(NS, NSL) * NSL;
+    constexpr int UF = ((2 * R + 1 + PD + UM - 1) / UM) * UM;
+    __shared__ vec lds[NSL][LH][LW];
 
     const int tid = threadIdx.x;
     const int tz = tid % NTZ, ty = tid / NTZ;
@@ -197,7 +238,7 @@ fd_step_kernel(const FdStepArgs a) {
     // streamed-once operands (P, C): optionally non-temporal so they do not
     // evict the U planes whose halos neighbouring tiles re-read from L2
     auto ldn = [](const float* base, uint32_t off) -> vec {
-        if constexpr (NTL) {
+        if constexpr ((NTL & FD_NT_LOADS) != 0) {
             using nv_t = typename Vec<VW>::N;
             const nv_t v = __builtin_nontemporal_load(reinterpret_cast<const nv_t*>(base + off));
             return V::from_native(v);
@@ -206,27 +247,52 @@ fd_step_kernel(const FdStepArgs a) {
         }
     };
 
-    // ---- warm-up: queue planes x0-R .. x0+R-1 into q[1..2R]
-    vec q[2 * R + 1][TYR];
+    // ---- prefetch ring.  Loads run PD planes ahead of the plane being
+    // computed: the U queue holds planes x-R .. x+R+PD (UF slots, plane p in
+    // slot (p - x0 + R) % UF), P and C are kept for planes x .. x+PD (NS
+    // slots, plane p in slot (p - x0) % NS).  The x loop is unrolled by UF, a
+    // multiple of NS and NSL, so every slot index is a compile-time constant:
+    // no register moves, and the wait for a plane's loads covers only that
+    // plane.  Halo rows and columns of a plane:
+    //   EH: fetched together with the plane's queue front (R planes before
+    //       it is the centre), when the neighbouring tiles fetch the same
+    //       cells as their own queue fronts, so they come from L2; held in
+    //       the plane's LDS buffer (NSL = R + 2 buffers, plane p in
+    //       (p - x0) % NSL) until it is the centre;
+    //   otherwise: fetched PD planes before the plane is the centre, two LDS
+    //       buffers.
+    vec q[UF][TYR], pq[NS][TYR], cq[NS][TYR], hq[NS][NH];
+    const int xlast = x1 - 1;
 #pragma unroll
-    for (int k = 1; k <= 2 * R; ++k) {
-        const uint32_t pb = (uint32_t)(x0 - R + k - 1) * plane;
+    for (int k = 0; k < 2 * R + PD; ++k) {
+        const int pl = min(x0 - R + k, xlast + R);   // beyond the chunk: in bounds, unused
+        const uint32_t pb = (uint32_t)pl * plane;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) q[k][r] = ldv(U, pb + roff[r]);
     }
-    // prefetch for plane x0
-    vec qn[TYR], pn[TYR], cn[TYR], hn[NH];
-    {
-        const uint32_t pb = (uint32_t)x0 * plane;
-        const uint32_t pf = (uint32_t)(x0 + R) * plane;
+    constexpr int HA = EH ? R : 0;   // halo planes lead the centre by HA
+    if constexpr (EH) {
+        // halos of planes x0 .. x0+R-1 go straight to their LDS buffers
+        // (ordered before the first reads by the first plane's barrier)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const uint32_t pb = (uint32_t)min(x0 + k, xlast) * plane;
+#pragma unroll
+            for (int m = 0; m < NH; ++m)
+                if (hslot[m] >= 0) (&lds[k % NSL][0][0])[hslot[m]] = ldv(U, pb + hoff[m]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PD; ++k) {
+        const uint32_t pb = (uint32_t)min(x0 + k, xlast) * plane;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
-            qn[r] = ldv(U, pf + roff[r]);
-            pn[r] = ldn(P, pb + roff[r]);
-            cn[r] = ldn(C, pb + roff[r]);
+            pq[k][r] = ldn(P, pb + roff[r]);
+            cq[k][r] = ldn(C, pb + roff[r]);
         }
+        const uint32_t hb = (uint32_t)min(x0 + HA + k, xlast) * plane;
 #pragma unroll
-        for (int m = 0; m < NH; ++m) hn[m] = ldv(U, pb + hoff[m]);
+        for (int m = 0; m < NH; ++m) hq[k][m] = ldv(U, hb + hoff[m]);
     }
 
     const float amp = a.amp;
@@ -234,38 +300,38 @@ fd_step_kernel(const FdStepArgs a) {
     const bool damp_next = a.damp_next != 0;
     bool guard_hit = false;
 
-    int it = 0;
-    for (int x = x0; x < x1; ++x, ++it) {
-        // rotate queue: q[k] <- q[k+1], q[2R] <- plane x+R
-#pragma unroll
-        for (int k = 0; k < 2 * R; ++k)
-#pragma unroll
-            for (int r = 0; r < TYR; ++r) q[k][r] = q[k + 1][r];
+    // one plane; ui (a compile-time constant) is x - xb with xb = x0 (mod UF)
+    auto plane_step = [&](auto uic, const int xb) __attribute__((always_inline)) {
+        constexpr int ui = decltype(uic)::value;
+        const int x = xb + ui;
+        // this plane's operands (compile-time slots: q[(ui + R + k) % UF] is plane x + k)
         vec pc[TYR], cc[TYR];
 #pragma unroll
-        for (int r = 0; r < TYR; ++r) { q[2 * R][r] = qn[r]; pc[r] = pn[r]; cc[r] = cn[r]; }
+        for (int r = 0; r < TYR; ++r) { pc[r] = pq[ui % NS][r]; cc[r] = cq[ui % NS][r]; }
 
-        // stage plane x in LDS
-        vec(*L)[LW] = lds[it & 1];
+        // stage plane x in LDS, and the halo of plane x + HA in its buffer
+        vec(*L)[LW] = lds[ui % NSL];
 #pragma unroll
-        for (int r = 0; r < TYR; ++r) L[R + ty * TYR + r][HV + tz] = q[R][r];
+        for (int r = 0; r < TYR; ++r) L[R + ty * TYR + r][HV + tz] = q[(ui + R) % UF][r];
 #pragma unroll
         for (int m = 0; m < NH; ++m)
-            if (hslot[m] >= 0) (&L[0][0])[hslot[m]] = hn[m];
+            if (hslot[m] >= 0) (&lds[(ui + HA) % NSL][0][0])[hslot[m]] = hq[ui % NS][m];
 
-        // prefetch plane x+1 (the last iteration re-reads plane x: in bounds, unused)
+        // prefetch plane x+PD and the halo of plane x+PD+HA (planes past
+        // the chunk re-read its last plane: in bounds, unused)
         {
-            const int xn = x + 1 < x1 ? x + 1 : x;
+            const int xn = min(x + PD, xlast);
             const uint32_t pb = (uint32_t)xn * plane;
             const uint32_t pf = (uint32_t)(xn + R) * plane;
 #pragma unroll
             for (int r = 0; r < TYR; ++r) {
-                qn[r] = ldv(U, pf + roff[r]);
-                pn[r] = ldn(P, pb + roff[r]);
-                cn[r] = ldn(C, pb + roff[r]);
+                q[(ui + 2 * R + PD) % UF][r] = ldv(U, pf + roff[r]);
+                pq[(ui + PD) % NS][r] = ldn(P, pb + roff[r]);
+                cq[(ui + PD) % NS][r] = ldn(C, pb + roff[r]);
             }
+            const uint32_t hb = (uint32_t)min(x + PD + HA, xlast) * plane;
 #pragma unroll
-            for (int m = 0; m < NH; ++m) hn[m] = ldv(U, pb + hoff[m]);
+            for (int m = 0; m < NH; ++m) hq[(ui + PD) % NS][m] = ldv(U, hb + hoff[m]);
         }
         __syncthreads();
 
@@ -284,29 +350,27 @@ fd_step_kernel(const FdStepArgs a) {
                 zw[h] = L[lr][tz + h];
                 zw[HV + 1 + h] = L[lr][HV + tz + 1 + h];
             }
-            zw[HV] = q[R][r];
+            zw[HV] = q[(ui + R) % UF][r];
             // y column
             vec yc[2 * R + 1];
 #pragma unroll
             for (int k = -R; k <= R; ++k) {
                 const int rr = r + k;
-                if (k == 0) yc[R] = q[R][r];
-                else if (rr >= 0 && rr < TYR) yc[R + k] = q[R][rr];
+                if (k == 0) yc[R] = q[(ui + R) % UF][r];
+                else if (rr >= 0 && rr < TYR) yc[R + k] = q[(ui + R) % UF][rr];
                 else yc[R + k] = L[lr + k][HV + tz];
             }
 #pragma unroll
             for (int j = 0; j < VW; ++j) {
-                const float u = V::get(q[R][r], j);
+                const float u = V::get(q[(ui + R) % UF][r], j);
                 float lap = 0.f;
 #pragma unroll
                 for (int k = 1; k <= R; ++k) {
-                    const float xs = (V::get(q[R + k][r], j) - u) + (V::get(q[R - k][r], j) - u);
-                    const float ys = (V::get(yc[R + k], j) - u) + (V::get(yc[R - k], j) - u);
                     const int zp = HV * VW + j + k, zm = HV * VW + j - k;
-                    const float zs = (V::get(zw[zp / VW], zp % VW) - u) + (V::get(zw[zm / VW], zm % VW) - u);
-                    lap = fmaf(a.cx[k], xs, lap);
-                    lap = fmaf(a.cy[k], ys, lap);
-                    lap = fmaf(a.cz[k], zs, lap);
+                    lap = lap_term(a.cx[k], V::get(q[(ui + R + k) % UF][r], j), V::get(q[(ui + R - k) % UF][r], j), u,
+                                   lap);
+                    lap = lap_term(a.cy[k], V::get(yc[R + k], j), V::get(yc[R - k], j), u, lap);
+                    lap = lap_term(a.cz[k], V::get(zw[zp / VW], zp % VW), V::get(zw[zm / VW], zm % VW), u, lap);
                 }
                 const float p = V::get(pc[r], j);
                 const float ds = fminf(dxs, dyz[r][j]);
@@ -316,37 +380,58 @@ fd_step_kernel(const FdStepArgs a) {
                 if (damp_next) nv *= ds;
                 if constexpr (MODE == FD_MODE_SNAPSHOT_LAP) V::set(lapv[r], j, lap);
                 bad |= !finitef(nv) && rvalid[r];
-                V::set(res[r], j, fminf(fmaxf(nv, -amp), amp));
+                // clamp to +-amp in one v_med3 (a non-finite nv is redone below)
+                V::set(res[r], j, __builtin_amdgcn_fmed3f(nv, -amp, amp));
             }
         }
         if (bad) {
             // rare path, reference guard order (wave_equation.cpp:345-397): a
             // non-finite centre or previous value gives 0, a non-finite
             // neighbour keeps the centre value (both unclamped), a non-finite
-            // result keeps the centre value (clamped).
+            // result keeps the centre value (clamped).  The neighbours come
+            // from the same registers and LDS plane as the fast path (no
+            // global loads here, so the prefetch ring's waits stay exact).
             guard_hit = true;
 #pragma unroll
             for (int r = 0; r < TYR; ++r) {
+                if (!rvalid[r]) continue;
+                const int lr = R + ty * TYR + r;
+                vec zw[2 * HV + 1];
+#pragma unroll
+                for (int h = 0; h < HV; ++h) {
+                    zw[h] = L[lr][tz + h];
+                    zw[HV + 1 + h] = L[lr][HV + tz + 1 + h];
+                }
+                zw[HV] = q[(ui + R) % UF][r];
+                vec yc[2 * R + 1];
+#pragma unroll
+                for (int k = -R; k <= R; ++k) {
+                    const int rr = r + k;
+                    if (k == 0) yc[R] = q[(ui + R) % UF][r];
+                    else if (rr >= 0 && rr < TYR) yc[R + k] = q[(ui + R) % UF][rr];
+                    else yc[R + k] = L[lr + k][HV + tz];
+                }
 #pragma unroll
                 for (int j = 0; j < VW; ++j) {
-                    const float u = V::get(q[R][r], j);
+                    const float u = V::get(q[(ui + R) % UF][r], j);
                     const float p = V::get(pc[r], j);
                     const float c = V::get(cc[r], j);
                     const float ds = fminf(dxs, dyz[r][j]);
                     const float d = damp_prev ? ds : 1.0f;
-                    if (!rvalid[r]) continue;
-                    const uint32_t ci = pb + roff[r] + j;
                     bool ok_n = true;
                     float lap = 0.f;
+#pragma unroll
                     for (int k = 1; k <= R; ++k) {
-                        const float xp = U[ci + k * plane], xm = U[ci - k * plane];
-                        const float yp = U[ci + k * nzh], ym = U[ci - k * nzh];
-                        const float zp = U[ci + k], zm = U[ci - k];
+                        const float xp = V::get(q[(ui + R + k) % UF][r], j);
+                        const float xm = V::get(q[(ui + R - k) % UF][r], j);
+                        const float yp = V::get(yc[R + k], j), ym = V::get(yc[R - k], j);
+                        const int zpi = HV * VW + j + k, zmi = HV * VW + j - k;
+                        const float zp = V::get(zw[zpi / VW], zpi % VW), zm = V::get(zw[zmi / VW], zmi % VW);
                         ok_n &= finitef(xp) && finitef(xm) && finitef(yp) && finitef(ym) && finitef(zp) &&
                                 finitef(zm);
-                        lap = fmaf(a.cx[k], (xp - u) + (xm - u), lap);
-                        lap = fmaf(a.cy[k], (yp - u) + (ym - u), lap);
-                        lap = fmaf(a.cz[k], (zp - u) + (zm - u), lap);
+                        lap = lap_term(a.cx[k], xp, xm, u, lap);
+                        lap = lap_term(a.cy[k], yp, ym, u, lap);
+                        lap = lap_term(a.cz[k], zp, zm, u, lap);
                     }
                     // Cerjan mode multiplies the reference result by D
                     const float dn = damp_next ? ds : 1.0f;
@@ -364,7 +449,7 @@ fd_step_kernel(const FdStepArgs a) {
 #pragma unroll
         for (int r = 0; r < TYR; ++r)
             if (rvalid[r]) {
-                if constexpr (NTL)
+                if constexpr ((NTL & FD_NT_STORE) != 0)
                     __builtin_nontemporal_store(V::to_native(res[r]),
                                                 reinterpret_cast<typename Vec<VW>::N*>(P + pb + roff[r]));
                 else *reinterpret_cast<vec*>(P + pb + roff[r]) = res[r];
@@ -376,7 +461,7 @@ fd_step_kernel(const FdStepArgs a) {
                     if constexpr (MODE == FD_MODE_SNAPSHOT_LAP)
                         __builtin_nontemporal_store(V::to_native(lapv[r]), dst);
                     else
-                        __builtin_nontemporal_store(V::to_native(q[R][r]), dst);
+                        __builtin_nontemporal_store(V::to_native(q[(ui + R) % UF][r]), dst);
                 }
                 if constexpr (MODE == FD_MODE_ACCUMULATE) {
                     // the snapshot is read once: non-temporal, like its store
@@ -402,7 +487,17 @@ fd_step_kernel(const FdStepArgs a) {
                     }
                 }
             }
-    }
+    };
+    // whole ring lengths run unguarded, so every wait on a plane's loads
+    // counts exactly the loads issued after them; the guarded tail (fewer
+    // than UF planes) is a second copy
+    int xb = x0;
+    for (; xb + UF <= x1; xb += UF)
+        static_for<UF>([&](auto uic) __attribute__((always_inline)) { plane_step(uic, xb); });
+    if (xb < x1)
+        static_for<UF>([&](auto uic) __attribute__((always_inline)) {
+            if (xb + decltype(uic)::value < x1) plane_step(uic, xb);   // uniform over the workgroup
+        });
 
     if (guard_hit && a.step_status) atomicOr(a.step_status, STP_STEP_NAN_DETECTED_AND_REPAIRED_DEV);
 
@@ -533,43 +628,43 @@ hipError_t fd_launch_stream3(int64_t n, const float* a, const float* b, const fl
 }
 
 // ------------------------------------------------------------------ launchers
-template <int R, int VW, int TZ, int TY, int TYR, int LX, bool NTL = false>
+template <int R, int VW, int TZ, int TY, int TYR, int LX, int NTL = 0, int PD = 1, bool EH = false>
 static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStream_t st) {
     constexpr int NT = (TZ / VW) * (TY / TYR);
     // the snapshot kind is a compile-time mode: a run-time choice between the
     // field and its Laplacian made the compiler spill the plane queue
     if (mode == FD_MODE_SNAPSHOT && a0.snap_lap)
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_SNAPSHOT_LAP>), dim3(nblocks), dim3(NT),
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_SNAPSHOT_LAP>), dim3(nblocks), dim3(NT),
                            0, st, a0);
     else if (mode == FD_MODE_SNAPSHOT)
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_SNAPSHOT>), dim3(nblocks), dim3(NT), 0,
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_SNAPSHOT>), dim3(nblocks), dim3(NT), 0,
                            st, a0);
     else if (mode == FD_MODE_ACCUMULATE)
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_ACCUMULATE>), dim3(nblocks), dim3(NT),
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_ACCUMULATE>), dim3(nblocks), dim3(NT),
                            0, st, a0);
     else
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, FD_MODE_PLAIN>), dim3(nblocks), dim3(NT), 0, st,
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_PLAIN>), dim3(nblocks), dim3(NT), 0, st,
                            a0);
     return hipGetLastError();
 }
 
-// Tile shapes (TZ, TY, TYR, LX, non-temporal P/C/N) compiled for the float4
-// path; STP_FD_TILE=<index> selects one.  Entry 0 (128 x 16 cells, one row
-// per thread, 512 threads, x chunks of 256 planes) is the default: at 512^3
-// it gives 256 workgroups, one per CU, and measured fastest (398 us/step).
-// Measured and not kept: loads issued two or three planes ahead (415-418 us),
-// non-temporal loads of P and C (430 us) or store of N (402 us), two 256-thread
-// workgroups per CU (128 x 8 or 64 x 16 cells: 413-419 us), shorter x chunks,
-// which re-read 2R halo planes of u^t per chunk (64 planes: 451 us), and taller
-// tiles (128 x 32, 256 x 16: 454-1171 us, or spills).  The others stay for
-// tuning on other grids.
-#define STP_FD_VARIANTS(X)            \
-    X(0, 128, 16, 1, 256, false)      \
-    X(1, 128, 16, 1, 128, false)      \
-    X(2, 128, 8, 1, 256, false)       \
-    X(3, 64, 16, 1, 256, false)       \
-    X(4, 128, 8, 1, 64, false)        \
-    X(5, 128, 16, 1, 256, true)
+// Tile shapes (TZ, TY, TYR, LX, non-temporal flags, prefetch distance in
+// planes, early halos) compiled for the float4 path; STP_FD_TILE=<index>
+// selects one.  Entry 0 is the default: 128 x 16 cells, one row per thread,
+// 512 threads, x chunks of 256 planes (at 512^3, 256 workgroups, one per CU),
+// loads two planes ahead, halos fetched with the queue front.  At 512^3 on
+// one box, interleaved: entry 0 392.2 us/step, 1 391.2, 2 394.2, 3 400.0,
+// 4 (the single-plane-ahead, late-halo kernel this replaced) 411.2, 5 420.1.
+// Measured earlier and not kept: shorter x chunks, which re-read 2R halo
+// planes of u^t per chunk (64 planes: 451 us), and taller tiles (128 x 32,
+// 256 x 16: 454-1171 us, or spills).
+#define STP_FD_VARIANTS(X)                \
+    X(0, 128, 16, 1, 256, 0, 2, true)     \
+    X(1, 128, 16, 1, 256, 3, 2, true)     \
+    X(2, 128, 16, 1, 256, 0, 3, true)     \
+    X(3, 128, 16, 1, 256, 0, 2, false)    \
+    X(4, 128, 16, 1, 256, 0, 1, false)    \
+    X(5, 128, 8, 1, 256, 0, 2, true)
 
 int fd_num_variants() { return 6; }
 
@@ -584,7 +679,7 @@ void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
     if (R == RR && VW == VV) { *tz = TZ; *ty = TY; *lx = LX; return; }
     STP_FD_FIXED(STP_FIXED_TILE)
 #undef STP_FIXED_TILE
-#define STP_TILE_CASE(id, TZ, TY, TYR, LX, NTL) \
+#define STP_TILE_CASE(id, TZ, TY, TYR, LX, NTL, PD, EH) \
     if (variant == id) { *tz = TZ; *ty = TY; *lx = LX; return; }
     STP_FD_VARIANTS(STP_TILE_CASE)
 #undef STP_TILE_CASE
@@ -598,8 +693,8 @@ hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, int m
     STP_FD_FIXED(STP_FIXED_LAUNCH)
 #undef STP_FIXED_LAUNCH
     if (R != 4 || VW != 4) return hipErrorInvalidValue;
-#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL) \
-    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL>(a, nblocks, mode, st);
+#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL, PD, EH) \
+    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL, PD, EH>(a, nblocks, mode, st);
     STP_FD_VARIANTS(STP_LAUNCH_CASE)
 #undef STP_LAUNCH_CASE
     return hipErrorInvalidValue;
