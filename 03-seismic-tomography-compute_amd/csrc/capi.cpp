@@ -129,6 +129,10 @@ int stp_fd_plan_create(const stp_fd_config* c, stp_fd_plan** out) {
     p->cfg = *c;
     p->R = R;
     p->VW = (c->halo_width % 4 == 0 && c->nz % 4 == 0) ? 4 : 1;
+    // One shot reads C once per step, so it is loaded non-temporally
+    // (variant 0); shots of a batch share C and read it again from the
+    // caches, so they keep it there (variant 1, the same tiling).
+    p->variant = c->nshots == 1 ? 0 : 1;
     // tile-shape override for tuning: STP_FD_TILE=<variant index>
     if (const char* ev = std::getenv("STP_FD_TILE")) {
         const int v = std::atoi(ev);

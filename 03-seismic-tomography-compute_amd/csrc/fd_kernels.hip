@@ -48,7 +48,7 @@
 namespace stp {
 
 // non-temporal access flags of the step (template parameter NTL)
-enum { FD_NT_LOADS = 1, FD_NT_STORE = 2 };
+enum { FD_NT_LOADS = 1, FD_NT_STORE = 2, FD_NT_COEF = 4 };
 
 template <int VW> struct Vec;
 typedef float f4n __attribute__((ext_vector_type(4)));
@@ -237,14 +237,17 @@ fd_step_kernel(const FdStepArgs a) {
     };
     // streamed-once operands (P, C): optionally non-temporal so they do not
     // evict the U planes whose halos neighbouring tiles re-read from L2
-    auto ldn = [](const float* base, uint32_t off) -> vec {
-        if constexpr ((NTL & FD_NT_LOADS) != 0) {
-            using nv_t = typename Vec<VW>::N;
-            const nv_t v = __builtin_nontemporal_load(reinterpret_cast<const nv_t*>(base + off));
-            return V::from_native(v);
-        } else {
-            return *reinterpret_cast<const vec*>(base + off);
-        }
+    auto ldnt = [](const float* base, uint32_t off) -> vec {
+        using nv_t = typename Vec<VW>::N;
+        return V::from_native(__builtin_nontemporal_load(reinterpret_cast<const nv_t*>(base + off)));
+    };
+    auto ldP = [&](const float* base, uint32_t off) -> vec {
+        if constexpr ((NTL & FD_NT_LOADS) != 0) return ldnt(base, off);
+        else return ldv(base, off);
+    };
+    auto ldC = [&](const float* base, uint32_t off) -> vec {
+        if constexpr ((NTL & (FD_NT_LOADS | FD_NT_COEF)) != 0) return ldnt(base, off);
+        else return ldv(base, off);
     };
 
     // ---- prefetch ring.  Loads run PD planes ahead of the plane being
@@ -287,8 +290,8 @@ fd_step_kernel(const FdStepArgs a) {
         const uint32_t pb = (uint32_t)min(x0 + k, xlast) * plane;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
-            pq[k][r] = ldn(P, pb + roff[r]);
-            cq[k][r] = ldn(C, pb + roff[r]);
+            pq[k][r] = ldP(P, pb + roff[r]);
+            cq[k][r] = ldC(C, pb + roff[r]);
         }
         const uint32_t hb = (uint32_t)min(x0 + HA + k, xlast) * plane;
 #pragma unroll
@@ -326,8 +329,8 @@ fd_step_kernel(const FdStepArgs a) {
 #pragma unroll
             for (int r = 0; r < TYR; ++r) {
                 q[(ui + 2 * R + PD) % UF][r] = ldv(U, pf + roff[r]);
-                pq[(ui + PD) % NS][r] = ldn(P, pb + roff[r]);
-                cq[(ui + PD) % NS][r] = ldn(C, pb + roff[r]);
+                pq[(ui + PD) % NS][r] = ldP(P, pb + roff[r]);
+                cq[(ui + PD) % NS][r] = ldC(C, pb + roff[r]);
             }
             const uint32_t hb = (uint32_t)min(x + PD + HA, xlast) * plane;
 #pragma unroll
@@ -650,23 +653,31 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
 
 // Tile shapes (TZ, TY, TYR, LX, non-temporal flags, prefetch distance in
 // planes, early halos) compiled for the float4 path; STP_FD_TILE=<index>
-// selects one.  Entry 0 is the default: 128 x 16 cells, one row per thread,
-// 512 threads, x chunks of 256 planes (at 512^3, 256 workgroups, one per CU),
-// loads two planes ahead, halos fetched with the queue front.  At 512^3 on
-// one box, interleaved: entry 0 392.2 us/step, 1 391.2, 2 394.2, 3 400.0,
-// 4 (the single-plane-ahead, late-halo kernel this replaced) 411.2, 5 420.1.
-// Measured earlier and not kept: shorter x chunks, which re-read 2R halo
-// planes of u^t per chunk (64 planes: 451 us), and taller tiles (128 x 32,
-// 256 x 16: 454-1171 us, or spills).
+// selects one.  Entry 0 is the default of one-shot plans: 128 x 16 cells,
+// one row per thread, 512 threads, x chunks of 256 planes (at 512^3, 256
+// workgroups, one per CU), loads two planes ahead, halos fetched with the
+// queue front, C (read once, never written) loaded non-temporally so it does
+// not take L2 lines from the u^t planes the neighbouring tiles share.  Entry
+// 1, the same without the non-temporal C, is the default of batched shots,
+// which share C and re-read it from the caches (16 shots at 256^3: 4.49 s
+// per gradient against 5.35 s with entry 0).  At 512^3, interleaved on one
+// box: entry 0 381.3 us/step, 1 389.0, 2 (three planes ahead) and 3 (P, C
+// and N all non-temporal) 388.5, 4 (late halos) ~398, 5 (the
+// single-plane-ahead, late-halo kernel this replaced) ~411; on another box
+// 384.3 and 387.6 us for entries 0 and 1.  Measured earlier and not kept:
+// shorter x chunks, which re-read 2R halo planes of u^t per chunk (64
+// planes: 451 us), taller tiles (128 x 32, 256 x 16: 454-1171 us, or
+// spills) and 256 x 8 cells (397 us).
 #define STP_FD_VARIANTS(X)                \
-    X(0, 128, 16, 1, 256, 0, 2, true)     \
-    X(1, 128, 16, 1, 256, 3, 2, true)     \
-    X(2, 128, 16, 1, 256, 0, 3, true)     \
-    X(3, 128, 16, 1, 256, 0, 2, false)    \
-    X(4, 128, 16, 1, 256, 0, 1, false)    \
-    X(5, 128, 8, 1, 256, 0, 2, true)
+    X(0, 128, 16, 1, 256, 4, 2, true)     \
+    X(1, 128, 16, 1, 256, 0, 2, true)     \
+    X(2, 128, 16, 1, 256, 4, 3, true)     \
+    X(3, 128, 16, 1, 256, 3, 2, true)     \
+    X(4, 128, 16, 1, 256, 0, 2, false)    \
+    X(5, 128, 16, 1, 256, 0, 1, false)    \
+    X(6, 128, 8, 1, 256, 4, 2, true)
 
-int fd_num_variants() { return 6; }
+int fd_num_variants() { return 7; }
 
 // Fixed tiles of the other paths (must match fd_launch_step below).
 #define STP_FD_FIXED(X)                 \
