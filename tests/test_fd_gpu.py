@@ -191,18 +191,28 @@ def test_invalid_arguments_raise(cuda):
         FDPropagator(16, 16, 16, order=4, device=cuda)
 
 
-def _run_fd(cuda, n, nt, chunks, single, nshots=1, seed=5):
+# Entries of STP_FD_VARIANTS in csrc/fd_kernels.hip (tile shape, prefetch
+# distance, early or late halos, non-temporal flags); STP_FD_TILE selects one
+# at plan creation.
+FD_VARIANTS = range(7)
+
+
+def _plan_with_variant(variant, *args, **kw):
     import os
-    import torch
-    old = os.environ.pop("STP_FD_PAIR", None)
-    if not single:
-        os.environ["STP_FD_PAIR"] = "1"
+    old = os.environ.pop("STP_FD_TILE", None)
+    if variant is not None:
+        os.environ["STP_FD_TILE"] = str(variant)
     try:
-        fd = FDPropagator(*n, order=8, sponge_width=6, nshots=nshots, device=cuda)
+        return FDPropagator(*args, **kw)
     finally:
-        os.environ.pop("STP_FD_PAIR", None)
+        os.environ.pop("STP_FD_TILE", None)
         if old is not None:
-            os.environ["STP_FD_PAIR"] = old
+            os.environ["STP_FD_TILE"] = old
+
+
+def _run_fd(cuda, n, nt, chunks, variant, nshots=1, seed=5):
+    import torch
+    fd = _plan_with_variant(variant, *n, order=8, sponge_width=6, nshots=nshots, device=cuda, sponge="cerjan")
     vel = synthetic.smooth_random_velocity(*n, seed=seed, sigma=3.0, vmin=1500.0, vmax=3500.0)
     fd.set_velocity(vel)
     srcs = [fd.index(n[0] // 2 + s, n[1] // 3, n[2] // 2 - s) for s in range(nshots)]
@@ -216,27 +226,29 @@ def _run_fd(cuda, n, nt, chunks, single, nshots=1, seed=5):
 
 
 @pytest.mark.parametrize("n,chunks,nshots", [
-    ((70, 45, 68), (40,), 1),          # partial tiles in y and z, even pair count
-    ((70, 45, 68), (6, 11, 23), 1),    # odd pair counts and an odd step count
-    ((36, 30, 28), (18,), 2),          # two shots
-    ((132, 33, 72), (10,), 1),         # two x chunks of the pair kernel
+    ((70, 45, 68), (40,), 1),          # one x chunk of 70 planes: the ring's guarded tail
+    ((70, 45, 68), (6, 11, 23), 1),    # the same steps split over three calls
+    ((36, 30, 28), (18,), 2),          # two shots (batched plans keep C cached)
+    ((300, 36, 136), (12,), 1),        # x chunks of 256 and 44 planes, both with tails
 ])
-def test_two_step_kernel_equals_single_steps(n, chunks, nshots, cuda):
-    """The pair kernel (temporal blocking) gives bit-identical fields and
-    traces to the single-step kernel, whatever the split of the steps."""
+def test_tile_variants_agree_bit_for_bit(n, chunks, nshots, cuda):
+    """Every tile variant (tile shape, loads one to three planes ahead, halos
+    with the queue front or with the centre plane, non-temporal accesses)
+    does the same arithmetic per cell: fields and traces are bit-identical,
+    on grids whose x chunks end in partial ring passes."""
     import torch
     nt = sum(chunks)
-    c1, p1, t1 = _run_fd(cuda, n, nt, chunks, single=True, nshots=nshots)
-    c2, p2, t2 = _run_fd(cuda, n, nt, chunks, single=False, nshots=nshots)
-    assert torch.equal(c1, c2)
-    assert torch.equal(t1, t2)
-    assert torch.equal(p1, p2)
+    ref = _run_fd(cuda, n, nt, chunks, None, nshots=nshots)
+    for v in FD_VARIANTS:
+        got = _run_fd(cuda, n, nt, chunks, v, nshots=nshots)
+        for a, b in zip(ref, got):
+            assert torch.equal(a, b), f"variant {v}"
 
 
-def test_two_step_kernel_guards_and_status(cuda):
-    """Non-finite values inside a pair: same guards and status words as the
-    oracle's step-by-step run (4 steps = two pair launches)."""
-    import os
+@pytest.mark.parametrize("variant", FD_VARIANTS)
+def test_tile_variants_guards_and_status(variant, cuda):
+    """Non-finite values: every variant applies the reference's guards and
+    reports the same step statuses as the oracle's step-by-step run."""
     import torch
     n = (20, 20, 20)
     hw = 4
@@ -251,11 +263,7 @@ def test_two_step_kernel_guards_and_status(cuda):
     v_h = np.full(nh, 2000.0)
     cur_o, prev_o = u.copy(), p.copy()
     s_o, _ = ofd.run(c, v_h, cur_o, prev_o, 4)
-    os.environ["STP_FD_PAIR"] = "1"
-    try:
-        fd = FDPropagator(*n, order=8, sponge_width=0, device=cuda)
-    finally:
-        os.environ.pop("STP_FD_PAIR", None)
+    fd = _plan_with_variant(variant, *n, order=8, sponge_width=0, device=cuda)
     fd.set_velocity(v_h)
     fd.cur[0] = torch.as_tensor(u, dtype=torch.float32)
     fd.prev[0] = torch.as_tensor(p, dtype=torch.float32)
