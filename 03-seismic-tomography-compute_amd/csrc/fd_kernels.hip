@@ -660,24 +660,22 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
 // not take L2 lines from the u^t planes the neighbouring tiles share.  Entry
 // 1, the same without the non-temporal C, is the default of batched shots,
 // which share C and re-read it from the caches (16 shots at 256^3: 4.49 s
-// per gradient against 5.35 s with entry 0).  At 512^3, interleaved on one
-// box: entry 0 381.3 us/step, 1 389.0, 2 (three planes ahead) and 3 (P, C
-// and N all non-temporal) 388.5, 4 (late halos) ~398, 5 (the
-// single-plane-ahead, late-halo kernel this replaced) ~411; on another box
-// 384.3 and 387.6 us for entries 0 and 1.  Measured earlier and not kept:
-// shorter x chunks, which re-read 2R halo planes of u^t per chunk (64
-// planes: 451 us), taller tiles (128 x 32, 256 x 16: 454-1171 us, or
-// spills) and 256 x 8 cells (397 us).
+// per gradient against 5.35 s with entry 0).  Entry 2 loads three planes
+// ahead; entry 3 is the single-plane-ahead, late-halo kernel this replaced,
+// kept as the reference point.  At 512^3, interleaved on one box: entry 0
+// 381.3 us/step, 1 389.0, 3 ~411; on another box 384.3, 387.6, 390.3 (entry
+// 2) and 411.7.  Measured and not kept: P, C and N all non-temporal (388.5,
+// against 389.0 for entry 1), late halos two planes ahead (398), 128 x 8
+// cells (422), 256 x 8 cells (397), shorter x chunks, which re-read 2R halo
+// planes of u^t per chunk (64 planes: 451 us), and taller tiles (128 x 32,
+// 256 x 16: 454-1171 us, or spills).
 #define STP_FD_VARIANTS(X)                \
     X(0, 128, 16, 1, 256, 4, 2, true)     \
     X(1, 128, 16, 1, 256, 0, 2, true)     \
     X(2, 128, 16, 1, 256, 4, 3, true)     \
-    X(3, 128, 16, 1, 256, 3, 2, true)     \
-    X(4, 128, 16, 1, 256, 0, 2, false)    \
-    X(5, 128, 16, 1, 256, 0, 1, false)    \
-    X(6, 128, 8, 1, 256, 4, 2, true)
+    X(3, 128, 16, 1, 256, 0, 1, false)
 
-int fd_num_variants() { return 7; }
+int fd_num_variants() { return 4; }
 
 // Fixed tiles of the other paths (must match fd_launch_step below).
 #define STP_FD_FIXED(X)                 \

@@ -194,7 +194,7 @@ def test_invalid_arguments_raise(cuda):
 # Entries of STP_FD_VARIANTS in csrc/fd_kernels.hip (tile shape, prefetch
 # distance, early or late halos, non-temporal flags); STP_FD_TILE selects one
 # at plan creation.
-FD_VARIANTS = range(7)
+FD_VARIANTS = range(4)
 
 
 def _plan_with_variant(variant, *args, **kw):
