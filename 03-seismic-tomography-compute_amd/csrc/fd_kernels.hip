@@ -26,7 +26,7 @@
 //     R planes before it is the centre, when the neighbouring tiles fetch
 //     the same cells as their own queue front, so they come from L2 (cut
 //     the step's DRAM reads from 1.18x to 1.05x the algorithmic bytes), and
-//     wait in that plane's LDS buffer (R+2 buffers);
+//     wait in that plane's LDS buffer (NB >= R+2 buffers);
 //   * workgroup ids are remapped so each XCD gets a contiguous range of tiles
 //     (neighbouring tiles share that XCD's L2 for their halos).
 // Sponge: D = min(Dx(plane), Dyz(cell)) because the reference factor is a
@@ -125,7 +125,7 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-template <int R, int VW, int TZ, int TY, int TYR, int LX, int NTL, int PD, bool EH, int MODE>
+template <int R, int VW, int TZ, int TY, int TYR, int LX, int NTL, int PD, int NB, int MODE>
 __global__ void __launch_bounds__((TZ / VW) * (TY / TYR))
 fd_step_kernel(const FdStepArgs a) {
     using V = Vec<VW>;
@@ -141,8 +141,11 @@ fd_step_kernel(const FdStepArgs a) {
 
     // ring lengths: NS register slots for P, C and halos (loads run PD
     // planes ahead), NSL LDS plane buffers, UF planes per unrolled pass
-    constexpr int NS = PD + 1;
-    constexpr int NSL = EH ? R + 2 : 2;
+    constexpr int NS = PD < 2 ? PD + 1 : (PD < 4 ? 4 : 8);   // >= PD + 1, a power of two
+    // NB > 0: halos fetched with the queue front (early), NB LDS plane buffers
+    constexpr bool EH = NB > 0;
+    static_assert(!EH || NB >= R + 2, "early halos need R + 2 LDS plane buffers");
+    constexpr int NSL = EH ? NB : 2;
     constexpr int UM = NS / gcd_c(NS, NSL) * NSL;
     constexpr int UF = ((2 * R + 1 + PD + UM - 1) / UM) * UM;
     __shared__ vec lds[NSL][LH][LW];
@@ -260,7 +263,7 @@ fd_step_kernel(const FdStepArgs a) {
     //   EH: fetched together with the plane's queue front (R planes before
     //       it is the centre), when the neighbouring tiles fetch the same
     //       cells as their own queue fronts, so they come from L2; held in
-    //       the plane's LDS buffer (NSL = R + 2 buffers, plane p in
+    //       the plane's LDS buffer (NSL = NB >= R + 2 buffers, plane p in
     //       (p - x0) % NSL) until it is the centre;
     //   otherwise: fetched PD planes before the plane is the centre, two LDS
     //       buffers.
@@ -631,49 +634,53 @@ hipError_t fd_launch_stream3(int64_t n, const float* a, const float* b, const fl
 }
 
 // ------------------------------------------------------------------ launchers
-template <int R, int VW, int TZ, int TY, int TYR, int LX, int NTL = 0, int PD = 1, bool EH = false>
+template <int R, int VW, int TZ, int TY, int TYR, int LX, int NTL = 0, int PD = 1, int NB = 0>
 static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStream_t st) {
     constexpr int NT = (TZ / VW) * (TY / TYR);
     // the snapshot kind is a compile-time mode: a run-time choice between the
     // field and its Laplacian made the compiler spill the plane queue
     if (mode == FD_MODE_SNAPSHOT && a0.snap_lap)
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_SNAPSHOT_LAP>), dim3(nblocks), dim3(NT),
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, NB, FD_MODE_SNAPSHOT_LAP>), dim3(nblocks), dim3(NT),
                            0, st, a0);
     else if (mode == FD_MODE_SNAPSHOT)
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_SNAPSHOT>), dim3(nblocks), dim3(NT), 0,
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, NB, FD_MODE_SNAPSHOT>), dim3(nblocks), dim3(NT), 0,
                            st, a0);
     else if (mode == FD_MODE_ACCUMULATE)
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_ACCUMULATE>), dim3(nblocks), dim3(NT),
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, NB, FD_MODE_ACCUMULATE>), dim3(nblocks), dim3(NT),
                            0, st, a0);
     else
-        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, EH, FD_MODE_PLAIN>), dim3(nblocks), dim3(NT), 0, st,
+        hipLaunchKernelGGL((fd_step_kernel<R, VW, TZ, TY, TYR, LX, NTL, PD, NB, FD_MODE_PLAIN>), dim3(nblocks), dim3(NT), 0, st,
                            a0);
     return hipGetLastError();
 }
 
 // Tile shapes (TZ, TY, TYR, LX, non-temporal flags, prefetch distance in
-// planes, early halos) compiled for the float4 path; STP_FD_TILE=<index>
+// planes, LDS plane buffers: 0 = halos fetched with the centre plane, else
+// with the queue front) compiled for the float4 path; STP_FD_TILE=<index>
 // selects one.  Entry 0 is the default of one-shot plans: 128 x 16 cells,
 // one row per thread, 512 threads, x chunks of 256 planes (at 512^3, 256
 // workgroups, one per CU), loads two planes ahead, halos fetched with the
 // queue front, C (read once, never written) loaded non-temporally so it does
 // not take L2 lines from the u^t planes the neighbouring tiles share.  Entry
 // 1, the same without the non-temporal C, is the default of batched shots,
-// which share C and re-read it from the caches (16 shots at 256^3: 4.49 s
+// which share C and re-read it from the caches (16 shots at 256^3: 4.37-4.49 s
 // per gradient against 5.35 s with entry 0).  Entry 2 loads three planes
-// ahead; entry 3 is the single-plane-ahead, late-halo kernel this replaced,
-// kept as the reference point.  At 512^3, interleaved on one box: entry 0
-// 381.3 us/step, 1 389.0, 3 ~411; on another box 384.3, 387.6, 390.3 (entry
-// 2) and 411.7.  Measured and not kept: P, C and N all non-temporal (388.5,
-// against 389.0 for entry 1), late halos two planes ahead (398), 128 x 8
-// cells (422), 256 x 8 cells (397), shorter x chunks, which re-read 2R halo
-// planes of u^t per chunk (64 planes: 451 us), and taller tiles (128 x 32,
-// 256 x 16: 454-1171 us, or spills).
+// ahead through 8 LDS buffers, whose 16-plane unrolled pass divides the
+// 256-plane chunk (no guarded tail); entry 3 is the single-plane-ahead,
+// late-halo kernel this replaced, kept as the reference point.  At 512^3,
+// interleaved on one box: entry 0 381.3 us/step, 1 389.0, 3 ~411; on others
+// 384.1-385.5 (entry 0) against 382.3-383.0 (entry 2), and 378.5-381.9
+// against 381.4-386.2 in full propagations: within the spread of the boxes.
+// Measured and not kept: three planes ahead with a guarded tail (390.3-390.8),
+// P, C and N all non-temporal (388.5, against 389.0 for entry 1), late halos
+// two planes ahead (398), 128 x 8 cells (422), 256 x 8 cells (397), shorter
+// x chunks, which re-read 2R halo planes of u^t per chunk (64 planes:
+// 451 us), and taller tiles (128 x 32, 256 x 16: 454-1171 us, or spills).
 #define STP_FD_VARIANTS(X)                \
-    X(0, 128, 16, 1, 256, 4, 2, true)     \
-    X(1, 128, 16, 1, 256, 0, 2, true)     \
-    X(2, 128, 16, 1, 256, 4, 3, true)     \
-    X(3, 128, 16, 1, 256, 0, 1, false)
+    X(0, 128, 16, 1, 256, 4, 2, 6)        \
+    X(1, 128, 16, 1, 256, 0, 2, 6)        \
+    X(2, 128, 16, 1, 256, 4, 3, 8)        \
+    X(3, 128, 16, 1, 256, 0, 1, 0)
 
 int fd_num_variants() { return 4; }
 
@@ -688,7 +695,7 @@ void fd_tiling(int R, int VW, int variant, int* tz, int* ty, int* lx) {
     if (R == RR && VW == VV) { *tz = TZ; *ty = TY; *lx = LX; return; }
     STP_FD_FIXED(STP_FIXED_TILE)
 #undef STP_FIXED_TILE
-#define STP_TILE_CASE(id, TZ, TY, TYR, LX, NTL, PD, EH) \
+#define STP_TILE_CASE(id, TZ, TY, TYR, LX, NTL, PD, NB) \
     if (variant == id) { *tz = TZ; *ty = TY; *lx = LX; return; }
     STP_FD_VARIANTS(STP_TILE_CASE)
 #undef STP_TILE_CASE
@@ -702,8 +709,8 @@ hipError_t fd_launch_step(const FdStepArgs& a, int R, int VW, int variant, int m
     STP_FD_FIXED(STP_FIXED_LAUNCH)
 #undef STP_FIXED_LAUNCH
     if (R != 4 || VW != 4) return hipErrorInvalidValue;
-#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL, PD, EH) \
-    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL, PD, EH>(a, nblocks, mode, st);
+#define STP_LAUNCH_CASE(id, TZ, TY, TYR, LX, NTL, PD, NB) \
+    if (variant == id) return launch_cfg<4, 4, TZ, TY, TYR, LX, NTL, PD, NB>(a, nblocks, mode, st);
     STP_FD_VARIANTS(STP_LAUNCH_CASE)
 #undef STP_LAUNCH_CASE
     return hipErrorInvalidValue;
