@@ -39,6 +39,8 @@ def _run_pair(n, order, hw, nt, sponge_width, vel, rcv, cuda, src=None, sponge="
 
 @pytest.mark.parametrize("n,order,hw", [
     ((40, 36, 44), 8, 4),    # float4 path, partial tiles on every axis
+    ((3, 30, 28), 8, 4),     # an x chunk shorter than the halo width and the prefetch distance
+    ((13, 18, 24), 8, 4),    # one full unrolled pass of 12 planes and a one-plane tail
     ((70, 33, 45), 8, 4),    # scalar path (nz % 4 != 0), two x chunks
     ((30, 20, 25), 2, 1),    # reference 7-point stencil, halo 1
     ((24, 20, 16), 2, 4),    # 7-point stencil on the float4 path
@@ -230,6 +232,7 @@ def _run_fd(cuda, n, nt, chunks, variant, nshots=1, seed=5):
     ((70, 45, 68), (6, 11, 23), 1),    # the same steps split over three calls
     ((36, 30, 28), (18,), 2),          # two shots (batched plans keep C cached)
     ((300, 36, 136), (12,), 1),        # x chunks of 256 and 44 planes, both with tails
+    ((3, 30, 28), (9,), 1),            # a chunk shorter than the halo width and every ring
 ])
 def test_tile_variants_agree_bit_for_bit(n, chunks, nshots, cuda):
     """Every tile variant (tile shape, loads one to three planes ahead, halos
