@@ -159,11 +159,11 @@ int stp_fd_plan_create(const stp_fd_config* c, stp_fd_plan** out) {
 
 int stp_fd_plan_destroy(stp_fd_plan* p) {
     if (!p) return STP_OK;
-    hipFree(p->d_blk_ptr);
-    hipFree(p->d_rcv_cell);
-    hipFree(p->d_rcv_id);
-    hipFree(p->d_src);
-    hipFree(p->d_word);
+    (void)hipFree(p->d_blk_ptr);
+    (void)hipFree(p->d_rcv_cell);
+    (void)hipFree(p->d_rcv_id);
+    (void)hipFree(p->d_src);
+    (void)hipFree(p->d_word);
     delete p;
     return STP_OK;
 }
@@ -199,7 +199,7 @@ int stp_fd_set_receivers(stp_fd_plan* p, const int64_t* idx, int32_t n) {
         cell[k] = idx[r];
         id[k] = r;
     }
-    hipFree(p->d_blk_ptr); hipFree(p->d_rcv_cell); hipFree(p->d_rcv_id);
+    (void)hipFree(p->d_blk_ptr); (void)hipFree(p->d_rcv_cell); (void)hipFree(p->d_rcv_id);
     p->d_blk_ptr = nullptr; p->d_rcv_cell = nullptr; p->d_rcv_id = nullptr;
     p->nrcv = 0; p->nrcv_pts = 0;
     if (n == 0) return STP_OK;
@@ -221,7 +221,7 @@ int stp_fd_set_sources(stp_fd_plan* p, const int64_t* idx, int32_t n) {
         if (!interior_coords(p, idx[s], &ix, &iy, &iz))
             return fail(STP_ERR_INVALID, "stp_fd_set_sources: source is not an interior cell");
     }
-    hipFree(p->d_src);
+    (void)hipFree(p->d_src);
     p->d_src = nullptr;
     STP_HIP(hipMalloc(&p->d_src, sizeof(int64_t) * n));
     STP_HIP(hipMemcpy(p->d_src, idx, sizeof(int64_t) * n, hipMemcpyHostToDevice));

@@ -161,7 +161,7 @@ int stp_csr_assemble(int64_t rows, int64_t cols, int64_t n, const int64_t* r, co
             m = (int64_t)hv;
             if (m > 0) {
                 hipLaunchKernelGGL(stp::asm_heads_kernel, dim3(stp::g256(m)), dim3(256), 0, st, m, k1, head);
-                hipFree(tmp);
+                (void)hipFree(tmp);
                 tmp = nullptr;
                 tb = 0;
                 if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, head, slot, (int)m, st)) != hipSuccess) break;
@@ -179,7 +179,7 @@ int stp_csr_assemble(int64_t rows, int64_t cols, int64_t n, const int64_t* r, co
         }
         hipLaunchKernelGGL(stp::u64_to_i64_kernel, dim3(stp::g256(rows + 1)), dim3(256), 0, st, rows + 1, rcnt, cnt64);
         if ((e = hipGetLastError()) != hipSuccess) break;
-        if (tmp) { hipFree(tmp); tmp = nullptr; }
+        if (tmp) { (void)hipFree(tmp); tmp = nullptr; }
         size_t tb = 0;
         if ((e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt64, row_ptr, (int)(rows + 1), st)) != hipSuccess)
             break;
@@ -187,8 +187,8 @@ int stp_csr_assemble(int64_t rows, int64_t cols, int64_t n, const int64_t* r, co
         if ((e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt64, row_ptr, (int)(rows + 1), st)) != hipSuccess) break;
         e = hipStreamSynchronize(st);
     } while (0);
-    hipFree(tmp); hipFree(k0); hipFree(k1); hipFree(p0); hipFree(p1); hipFree(head); hipFree(slot);
-    hipFree(valid); hipFree(rcnt); hipFree(cnt64);
+    (void)hipFree(tmp); (void)hipFree(k0); (void)hipFree(k1); (void)hipFree(p0); (void)hipFree(p1); (void)hipFree(head); (void)hipFree(slot);
+    (void)hipFree(valid); (void)hipFree(rcnt); (void)hipFree(cnt64);
     if (e != hipSuccess) return stp::hip_fail(e, "stp_csr_assemble");
     *nnz_host = nnz;
     return STP_OK;
@@ -235,7 +235,7 @@ int stp_dot_f64(int64_t n, const double* a, const double* b, double* out_host, v
     double* hp = new double[nb];
     if (e == hipSuccess) e = hipMemcpyAsync(hp, part, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(part);
+    (void)hipFree(part);
     double s = 0.0;
     for (unsigned i = 0; i < nb; ++i) s += hp[i];
     delete[] hp;

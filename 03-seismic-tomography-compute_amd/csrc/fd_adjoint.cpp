@@ -31,7 +31,7 @@ namespace {
 
 struct DevBuf {
     void* p = nullptr;
-    ~DevBuf() { if (p) hipFree(p); }
+    ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
 // e = d - o (FP32) and per-block FP64 partial sums of e^2

@@ -116,7 +116,7 @@ hipError_t health_count(const double* d, int64_t n, int64_t* out_host, hipStream
             (e = hipMemcpyAsync(&h, dc, sizeof(h), hipMemcpyDeviceToHost, st)) == hipSuccess)
             e = hipStreamSynchronize(st);
     }
-    hipFree(dc);
+    (void)hipFree(dc);
     *out_host = (int64_t)h;
     return e;
 }
@@ -159,10 +159,10 @@ hipError_t health_repair(double* d, int64_t count, int64_t nx, int64_t ny, int64
                     if (e == hipSuccess) e = hipStreamSynchronize(st);
                 }
             }
-            if (tmp) hipFree(tmp);
-            if (l1) hipFree(l1);
-            if (l0) hipFree(l0);
-            if (dc) hipFree(dc);
+            if (tmp) (void)hipFree(tmp);
+            if (l1) (void)hipFree(l1);
+            if (l0) (void)hipFree(l0);
+            if (dc) (void)hipFree(dc);
             if (e != hipSuccess) return e;
         }
     }
@@ -182,7 +182,7 @@ hipError_t field_energy_f32(const float* u, int64_t n, double* out_host, hipStre
     double hp[1024];
     if (e == hipSuccess) e = hipMemcpyAsync(hp, part, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(part);
+    (void)hipFree(part);
     if (e != hipSuccess) return e;
     double sum = 0.0;
     for (unsigned k = 0; k < nb; ++k) sum += hp[k];

@@ -363,7 +363,7 @@ void free_grid(stp_lpc* h) {
                   h->e,    h->out,  h->coef, h->dsq, h->nloc,  h->dacc, h->de,   h->dcat1, h->dd,
                   h->da,   h->dcat2, h->dc,  h->db,  h->dpb,   h->dpa,  h->dO,   h->dpre};
     for (void* p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     h->feat = h->a = h->pa = h->b = h->pb = h->c = h->cat2 = h->d = h->cat1 = h->e = nullptr;
     h->out = h->coef = h->dsq = nullptr;
     h->nloc = nullptr;
@@ -380,7 +380,7 @@ void free_all(stp_lpc* h) {
     free_grid(h);
     void* ps[] = {h->w32, h->wbf, h->wflip, h->grad, h->m1, h->m2, h->wdw};
     for (void* p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
 }
 
 int ensure_grid(stp_lpc* h, const int64_t* n, bool train) {
@@ -731,7 +731,7 @@ int stp_conv3d_bf16(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t cou
         e = stp::conv3d_fwd(stp::ConvShape{(int)n0, (int)n1, (int)n2, cin, cout}, in_dev, wb, relu, stp::CONV_OUT_F32,
                             out_dev, cout, 0, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(wb);
+    (void)hipFree(wb);
     if (e != hipSuccess) return stp::hip_fail(e, "stp_conv3d_bf16");
     return STP_OK;
 }
@@ -749,7 +749,7 @@ int stp_conv3d_bf16_bwd(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t
         e = dalloc(&work, std::max<size_t>(stp::conv3d_bwd_weight_work(s), 1));
         if (e == hipSuccess) e = stp::conv3d_bwd_weight(s, in_dev, dout_dev, dw_dev, work, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (work) hipFree(work);
+        if (work) (void)hipFree(work);
     }
     if (e == hipSuccess && din_dev) {
         bf16_t* wb = nullptr;
@@ -759,7 +759,7 @@ int stp_conv3d_bf16_bwd(int64_t n0, int64_t n1, int64_t n2, int32_t cin, int32_t
             e = stp::conv3d_fwd(stp::ConvShape{(int)n0, (int)n1, (int)n2, cout, cin}, dout_dev, wb, 0,
                                 stp::CONV_OUT_F32, din_dev, cin, 0, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (wb) hipFree(wb);
+        if (wb) (void)hipFree(wb);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return stp::hip_fail(e, "stp_conv3d_bf16_bwd");

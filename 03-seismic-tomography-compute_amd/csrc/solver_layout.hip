@@ -162,7 +162,7 @@ hipError_t sort_pairs(K* k0, K* k1, int* v0, int* v1, int n, int end_bit, hipStr
     if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, n, 0, end_bit, st);
     if (tmp) {
         const hipError_t e2 = hipStreamSynchronize(st);
-        hipFree(tmp);
+        (void)hipFree(tmp);
         if (e == hipSuccess) e = e2;
     }
     return e;
@@ -191,7 +191,7 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
         if ((e = hipMalloc(&uidx, sizeof(int) * ((size_t)nnz + 1))) != hipSuccess) break;
         for (int rt = 64; rt >= 32; rt /= 2) {
             const int nt = (n + rt - 1) / rt;
-            hipFree(tu);
+            (void)hipFree(tu);
             tu = nullptr;
             if ((e = hipMalloc(&tu, sizeof(int) * ((size_t)nt + 1))) != hipSuccess) break;
             hipLaunchKernelGGL(stage_key_kernel, dim3(blocks((int64_t)n * 64)), dim3(256), 0, st, n, h->ltrp, h->ltci,
@@ -227,7 +227,7 @@ hipError_t build_staged_transpose(stp_csr* h, hipStream_t st) {
         }
         if (e == hipSuccess) e = hipStreamSynchronize(st);
     } while (0);
-    hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(flag); hipFree(uidx); hipFree(tu);
+    (void)hipFree(k0); (void)hipFree(k1); (void)hipFree(i0); (void)hipFree(i1); (void)hipFree(flag); (void)hipFree(uidx); (void)hipFree(tu);
     return e;
 }
 
@@ -237,7 +237,7 @@ void free_solver_layout(stp_csr* h) {
     void* ps[] = {h->cperm, h->rsrc, h->lrp, h->lci, h->lval, h->ltrp, h->ltci, h->ltval, h->lx, h->lb,
                   h->st_tu, h->st_uniq, h->st_l16};
     for (void* p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     h->cperm = h->rsrc = h->lrp = h->lci = h->ltrp = h->ltci = nullptr;
     h->lval = h->ltval = nullptr;
     h->lx = h->lb = nullptr;
@@ -308,7 +308,7 @@ int build_solver_layout(stp_csr* h, hipStream_t st) {
         if ((e = build_staged_transpose(h, st)) != hipSuccess) break;
         e = hipStreamSynchronize(st);
     } while (0);
-    hipFree(k0); hipFree(k1); hipFree(i0); hipFree(i1); hipFree(len);
+    (void)hipFree(k0); (void)hipFree(k1); (void)hipFree(i0); (void)hipFree(i1); (void)hipFree(len);
     if (e != hipSuccess) {
         free_solver_layout(h);
         return hip_fail(e, "solver layout");

@@ -36,7 +36,7 @@ constexpr int STALL = 512;
 template <typename T>
 hipError_t grow(T** p, size_t* cap, size_t n) {
     if (*cap >= n && *p) return hipSuccess;
-    if (*p) hipFree(*p);
+    if (*p) (void)hipFree(*p);
     *p = nullptr;
     *cap = 0;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1));
@@ -50,7 +50,7 @@ static void csr_free(stp_csr* h) {
     void* ps[] = {h->rp, h->ci, h->trp, h->tci, h->val, h->tval, h->val64, h->r, h->part0, h->part1, h->part2, h->part3, h->part4, h->part5,
                   h->p, h->q, h->s, h->z, h->sp, h->dinv, h->r32, h->ry, h->rg, h->pnorm, h->dp, h->dap, h->dz, h->dr, h->ddinv, h->sc};
     for (void* p : ps)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     stp::free_solver_layout(h);
 }
 
@@ -122,7 +122,7 @@ int stp_csr_from_rays(int64_t nrays, const int64_t* n, double h_cell, const doub
         if ((e = stp::rays_fill(nrays, pts, n, h_cell, h->rp, h->ci, h->val, st)) != hipSuccess) break;
         e = hipStreamSynchronize(st);
     } while (0);
-    if (counts) hipFree(counts);
+    if (counts) (void)hipFree(counts);
     if (e != hipSuccess) { csr_free(h); delete h; return hip_fail(e, "stp_csr_from_rays"); }
     const int rc = finish_csr(h, st);
     if (rc != STP_OK) { csr_free(h); delete h; return rc; }
@@ -244,7 +244,7 @@ int stp_csr_gather_probe(const stp_csr* h, int32_t transpose, int32_t fp64, cons
 int stp_csr_set_values_f64(stp_csr* h, const double* val64, void* stream) {
     if (!h) return fail(STP_ERR_INVALID, "stp_csr_set_values_f64: null handle");
     if (!val64) {
-        if (h->val64) hipFree(h->val64);
+        if (h->val64) (void)hipFree(h->val64);
         h->val64 = nullptr;
         return STP_OK;
     }

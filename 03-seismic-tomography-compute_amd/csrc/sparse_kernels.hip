@@ -852,7 +852,7 @@ hipError_t exclusive_scan_i32(const int* in, int* out, int n, hipStream_t st) {
     if ((e = hipMalloc(&tmp, tb ? tb : 1)) != hipSuccess) return e;
     e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(tmp);
+    (void)hipFree(tmp);
     return e;
 }
 
@@ -885,8 +885,11 @@ hipError_t csr_transpose(int rows, int cols, int64_t nnz, const int* rp, const i
         }
         if ((e = exclusive_scan_i32(counts, trp, cols + 1, st)) != hipSuccess) break;
     } while (0);
-    hipStreamSynchronize(st);
-    hipFree(tmp); hipFree(k0); hipFree(k1); hipFree(v1); hipFree(counts);
+    {   // report a failure of the queued work unless an earlier one is reported
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
+    }
+    (void)hipFree(tmp); (void)hipFree(k0); (void)hipFree(k1); (void)hipFree(v1); (void)hipFree(counts);
     return e;
 }
 
