@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import math
 import time
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -28,52 +29,59 @@ def _torch():
     return torch
 
 
-class FaultRecoveryPolicy:
-    """inversion.py:20-88: counters, the NaN-ratio threshold, step reduction
-    and the velocity clamp."""
+_FAULT_COUNTERS = ("total_faults", "consecutive_faults", "recovered_iterations", "skipped_iterations")
 
-    def __init__(self, max_consecutive_faults: int = 5, nan_ratio_threshold: float = 0.1,
-                 step_reduction_factor: float = 0.5, min_step_length: float = 1e-8,
-                 velocity_clamp_range: tuple = (1450.0, 8500.0)):
-        self.max_consecutive_faults = max_consecutive_faults
-        self.nan_ratio_threshold = nan_ratio_threshold
-        self.step_reduction_factor = step_reduction_factor
-        self.min_step_length = min_step_length
-        self.velocity_clamp_range = velocity_clamp_range
-        self._consecutive_faults = 0
-        self._total_faults = 0
-        self._recovered_iterations = 0
-        self._skipped_iterations = 0
+
+@dataclass
+class FaultRecoveryPolicy:
+    """Fault bookkeeping of the inversion loop (inversion.py:20-88): when to
+    give up, how far to shrink the step after a failed update, and the
+    velocity bounds a model is clamped into.  The constructor arguments,
+    the method names and the keys of `stats` are the reference's, so code
+    that configures or queries the policy keeps working; the rules are:
+    stop once `max_consecutive_faults` faults follow one another or when
+    more than `nan_ratio_threshold` of the model is non-finite; a reduced
+    step is the step times `step_reduction_factor`, never below
+    `min_step_length`; non-finite velocities become the lower bound."""
+
+    max_consecutive_faults: int = 5
+    nan_ratio_threshold: float = 0.1
+    step_reduction_factor: float = 0.5
+    min_step_length: float = 1e-8
+    velocity_clamp_range: tuple = (1450.0, 8500.0)
+    _count: dict = field(default_factory=lambda: dict.fromkeys(_FAULT_COUNTERS, 0), init=False, repr=False)
+
+    def _bump(self, *names: str) -> None:
+        for n in names:
+            self._count[n] += 1
 
     def should_continue(self, nan_ratio: float) -> bool:
-        if self._consecutive_faults >= self.max_consecutive_faults:
-            return False
-        return not nan_ratio > self.nan_ratio_threshold
+        too_many = self._count["consecutive_faults"] >= self.max_consecutive_faults
+        return not too_many and not nan_ratio > self.nan_ratio_threshold
 
-    def record_fault(self):
-        self._consecutive_faults += 1
-        self._total_faults += 1
+    def record_fault(self) -> None:
+        self._bump("consecutive_faults", "total_faults")
 
-    def record_recovery(self):
-        self._recovered_iterations += 1
-        self._consecutive_faults = 0
+    def record_recovery(self) -> None:
+        self._bump("recovered_iterations")
+        self._count["consecutive_faults"] = 0
 
-    def record_skip(self):
-        self._skipped_iterations += 1
+    def record_skip(self) -> None:
+        self._bump("skipped_iterations")
 
     def reduced_step_length(self, current_step: float) -> float:
         return max(current_step * self.step_reduction_factor, self.min_step_length)
 
     def clamp_velocity(self, vp):
+        """Non-finite entries to the lower bound, then every entry into the
+        bounds (a torch tensor, on its device)."""
         torch = _torch()
         lo, hi = self.velocity_clamp_range
-        vp = torch.where(torch.isfinite(vp), vp, torch.full_like(vp, lo))
-        return vp.clamp(lo, hi)
+        return torch.nan_to_num(vp, nan=lo, posinf=lo, neginf=lo).clamp(lo, hi)
 
     @property
     def stats(self) -> dict:
-        return {"total_faults": self._total_faults, "consecutive_faults": self._consecutive_faults,
-                "recovered_iterations": self._recovered_iterations, "skipped_iterations": self._skipped_iterations}
+        return {n: self._count[n] for n in _FAULT_COUNTERS}
 
 
 class TravelTimeInversion:
