@@ -138,6 +138,7 @@ int stp_fd_plan_create(const stp_fd_config* c, stp_fd_plan** out) {
         const int v = std::atoi(ev);
         if (v >= 0 && v < stp::fd_num_variants()) p->variant = v;
     }
+    if (const char* ev = std::getenv("STP_FD_SWEEP")) p->alt_sweep = std::strcmp(ev, "up") != 0;
     stp::fd_tiling(R, p->VW, p->variant, &p->tz, &p->ty, &p->lx);
     p->nxh = (int)nxh; p->nyh = (int)nyh; p->nzh = (int)nzh;
     p->ncell = nxh * nyh * nzh;
@@ -309,6 +310,7 @@ int stp_fd_propagate(stp_fd_plan* p, float* cur, float* prev, const float* coef,
         a.t_col = s;
         a.w_next = (inject && s + 1 < nsteps && t + 1 < nw) ? wavelet + (t + 1) : nullptr;
         a.step_status = status ? status + s : nullptr;
+        a.rev = p->alt_sweep ? t & 1 : 0;   // alternate sweeps: start on the planes the last step left in the caches
         STP_HIP(stp::fd_launch_step(a, p->R, p->VW, p->variant, stp::FD_MODE_PLAIN, S(st)));
         std::swap(u, v);
         return STP_OK;

@@ -79,6 +79,7 @@ int forward_steps(stp_fd_plan* p, float** u, float** v, const float* coef, int n
         a.t_col = s;
         a.w_next = (inject && s + 1 < nsteps && t + 1 < nw) ? wavelet + (t + 1) : nullptr;
         a.snap_out = snap ? snap + s * slot : nullptr;
+        a.rev = p->alt_sweep ? t & 1 : 0;   // alternate sweeps (same bits either way)
         STP_HIP(stp::fd_launch_step(a, p->R, p->VW, p->variant,
                                     snap ? stp::FD_MODE_SNAPSHOT : stp::FD_MODE_PLAIN, st));
         std::swap(*u, *v);
@@ -189,6 +190,7 @@ int stp_fd_adjoint(stp_fd_plan* p, const float* coef, const float* wavelet, int3
             a.uprev = av;
             a.acc_src = sg + (int64_t)(t - t0) * field;
             a.inj_col = t;
+            a.rev = p->alt_sweep ? t & 1 : 0;
             STP_HIP(stp::fd_launch_step(a, p->R, p->VW, p->variant, stp::FD_MODE_ACCUMULATE, st));
             std::swap(au, av);
         }

@@ -35,6 +35,7 @@ struct FdStepArgs {
     const float* w_next;      // &wavelet[t+1] of shot 0; shot s at s*w_stride
     int64_t w_stride;
     int* step_status;         // OR-ed with STP_STEP_NAN_DETECTED_AND_REPAIRED
+    int rev;                  // march each x chunk from its last plane to its first
     // mode 1 (forward recompute): per-shot with-halo snapshot of the step's
     // input field (snap_lap = 0) or of its Laplacian L(u) (snap_lap = 1)
     float* snap_out;

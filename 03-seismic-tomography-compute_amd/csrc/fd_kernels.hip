@@ -28,7 +28,10 @@
 //     the step's DRAM reads from 1.18x to 1.05x the algorithmic bytes), and
 //     wait in that plane's LDS buffer (NB >= R+2 buffers);
 //   * workgroup ids are remapped so each XCD gets a contiguous range of tiles
-//     (neighbouring tiles share that XCD's L2 for their halos).
+//     (neighbouring tiles share that XCD's L2 for their halos);
+//   * a.rev marches each x chunk from its last plane down: the host
+//     alternates it step by step, so a step starts on the planes the previous
+//     one touched last, part of which the Infinity Cache still holds.
 // Sponge: D = min(Dx(plane), Dyz(cell)) because the reference factor is a
 // monotone function of min(distance) — Dyz is computed once per thread.
 // Cerjan mode (damp_next, not in the reference) multiplies the result by D as
@@ -254,24 +257,33 @@ fd_step_kernel(const FdStepArgs a) {
     };
 
     // ---- prefetch ring.  Loads run PD planes ahead of the plane being
-    // computed: the U queue holds planes x-R .. x+R+PD (UF slots, plane p in
-    // slot (p - x0 + R) % UF), P and C are kept for planes x .. x+PD (NS
-    // slots, plane p in slot (p - x0) % NS).  The x loop is unrolled by UF, a
-    // multiple of NS and NSL, so every slot index is a compile-time constant:
+    // computed (planes counted within the chunk, see px below): the U queue
+    // holds planes i-R .. i+R+PD (UF slots, plane p in slot (p + R) % UF), P
+    // and C are kept for planes i .. i+PD (NS slots, plane p in slot p % NS).
+    // The x loop is unrolled by UF, a multiple of NS and NSL, so every slot
+    // index is a compile-time constant:
     // no register moves, and the wait for a plane's loads covers only that
     // plane.  Halo rows and columns of a plane:
     //   EH: fetched together with the plane's queue front (R planes before
     //       it is the centre), when the neighbouring tiles fetch the same
     //       cells as their own queue fronts, so they come from L2; held in
     //       the plane's LDS buffer (NSL = NB >= R + 2 buffers, plane p in
-    //       (p - x0) % NSL) until it is the centre;
+    //       p % NSL) until it is the centre;
     //   otherwise: fetched PD planes before the plane is the centre, two LDS
     //       buffers.
     vec q[UF][TYR], pq[NS][TYR], cq[NS][TYR], hq[NS][NH];
-    const int xlast = x1 - 1;
+    // Planes are numbered within the chunk (0 .. llast, and R past either
+    // end for the queue); a.rev marches the chunk from its last plane down,
+    // so a step starts on the planes the previous one touched last.  The x
+    // taps enter the Laplacian as (u[x+k] - u) + (u[x-k] - u), whose value
+    // does not depend on the order of the two terms, so both directions give
+    // the same bits.
+    const int llast = x1 - x0 - 1;
+    const bool rev = a.rev != 0;
+    auto px = [&](int i) -> int { return rev ? x1 - 1 - i : x0 + i; };
 #pragma unroll
     for (int k = 0; k < 2 * R + PD; ++k) {
-        const int pl = min(x0 - R + k, xlast + R);   // beyond the chunk: in bounds, unused
+        const int pl = px(min(k - R, llast + R));   // beyond the chunk: in bounds, unused
         const uint32_t pb = (uint32_t)pl * plane;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) q[k][r] = ldv(U, pb + roff[r]);
@@ -282,7 +294,7 @@ fd_step_kernel(const FdStepArgs a) {
         // (ordered before the first reads by the first plane's barrier)
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const uint32_t pb = (uint32_t)min(x0 + k, xlast) * plane;
+            const uint32_t pb = (uint32_t)px(min(k, llast)) * plane;
 #pragma unroll
             for (int m = 0; m < NH; ++m)
                 if (hslot[m] >= 0) (&lds[k % NSL][0][0])[hslot[m]] = ldv(U, pb + hoff[m]);
@@ -290,13 +302,13 @@ fd_step_kernel(const FdStepArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < PD; ++k) {
-        const uint32_t pb = (uint32_t)min(x0 + k, xlast) * plane;
+        const uint32_t pb = (uint32_t)px(min(k, llast)) * plane;
 #pragma unroll
         for (int r = 0; r < TYR; ++r) {
             pq[k][r] = ldP(P, pb + roff[r]);
             cq[k][r] = ldC(C, pb + roff[r]);
         }
-        const uint32_t hb = (uint32_t)min(x0 + HA + k, xlast) * plane;
+        const uint32_t hb = (uint32_t)px(min(HA + k, llast)) * plane;
 #pragma unroll
         for (int m = 0; m < NH; ++m) hq[k][m] = ldv(U, hb + hoff[m]);
     }
@@ -306,16 +318,18 @@ fd_step_kernel(const FdStepArgs a) {
     const bool damp_next = a.damp_next != 0;
     bool guard_hit = false;
 
-    // one plane; ui (a compile-time constant) is x - xb with xb = x0 (mod UF)
-    auto plane_step = [&](auto uic, const int xb) __attribute__((always_inline)) {
+    // one plane: chunk plane i = ib + ui (ui a compile-time constant, ib a
+    // multiple of UF), grid plane x
+    auto plane_step = [&](auto uic, const int ib) __attribute__((always_inline)) {
         constexpr int ui = decltype(uic)::value;
-        const int x = xb + ui;
-        // this plane's operands (compile-time slots: q[(ui + R + k) % UF] is plane x + k)
+        const int i = ib + ui;
+        const int x = px(i);
+        // this plane's operands (compile-time slots: q[(ui + R + k) % UF] is chunk plane i + k)
         vec pc[TYR], cc[TYR];
 #pragma unroll
         for (int r = 0; r < TYR; ++r) { pc[r] = pq[ui % NS][r]; cc[r] = cq[ui % NS][r]; }
 
-        // stage plane x in LDS, and the halo of plane x + HA in its buffer
+        // stage plane i in LDS, and the halo of plane i + HA in its buffer
         vec(*L)[LW] = lds[ui % NSL];
 #pragma unroll
         for (int r = 0; r < TYR; ++r) L[R + ty * TYR + r][HV + tz] = q[(ui + R) % UF][r];
@@ -323,19 +337,19 @@ fd_step_kernel(const FdStepArgs a) {
         for (int m = 0; m < NH; ++m)
             if (hslot[m] >= 0) (&lds[(ui + HA) % NSL][0][0])[hslot[m]] = hq[ui % NS][m];
 
-        // prefetch plane x+PD and the halo of plane x+PD+HA (planes past
+        // prefetch plane i+PD and the halo of plane i+PD+HA (planes past
         // the chunk re-read its last plane: in bounds, unused)
         {
-            const int xn = min(x + PD, xlast);
-            const uint32_t pb = (uint32_t)xn * plane;
-            const uint32_t pf = (uint32_t)(xn + R) * plane;
+            const int in = min(i + PD, llast);
+            const uint32_t pb = (uint32_t)px(in) * plane;
+            const uint32_t pf = (uint32_t)px(in + R) * plane;
 #pragma unroll
             for (int r = 0; r < TYR; ++r) {
                 q[(ui + 2 * R + PD) % UF][r] = ldv(U, pf + roff[r]);
                 pq[(ui + PD) % NS][r] = ldP(P, pb + roff[r]);
                 cq[(ui + PD) % NS][r] = ldC(C, pb + roff[r]);
             }
-            const uint32_t hb = (uint32_t)min(x + PD + HA, xlast) * plane;
+            const uint32_t hb = (uint32_t)px(min(i + PD + HA, llast)) * plane;
 #pragma unroll
             for (int m = 0; m < NH; ++m) hq[(ui + PD) % NS][m] = ldv(U, hb + hoff[m]);
         }
@@ -497,12 +511,13 @@ fd_step_kernel(const FdStepArgs a) {
     // whole ring lengths run unguarded, so every wait on a plane's loads
     // counts exactly the loads issued after them; the guarded tail (fewer
     // than UF planes) is a second copy
-    int xb = x0;
-    for (; xb + UF <= x1; xb += UF)
-        static_for<UF>([&](auto uic) __attribute__((always_inline)) { plane_step(uic, xb); });
-    if (xb < x1)
+    const int xlen = llast + 1;
+    int ib = 0;
+    for (; ib + UF <= xlen; ib += UF)
+        static_for<UF>([&](auto uic) __attribute__((always_inline)) { plane_step(uic, ib); });
+    if (ib < xlen)
         static_for<UF>([&](auto uic) __attribute__((always_inline)) {
-            if (xb + decltype(uic)::value < x1) plane_step(uic, xb);   // uniform over the workgroup
+            if (ib + decltype(uic)::value < xlen) plane_step(uic, ib);   // uniform over the workgroup
         });
 
     if (guard_hit && a.step_status) atomicOr(a.step_status, STP_STEP_NAN_DETECTED_AND_REPAIRED_DEV);

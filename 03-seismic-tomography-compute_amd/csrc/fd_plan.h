@@ -8,6 +8,7 @@
 struct stp_fd_plan {
     stp_fd_config cfg;
     int R = 0, VW = 1, variant = 0, tz = 0, ty = 0, lx = 0, ntz = 0, nty = 0, nxc = 0;
+    int alt_sweep = 1;   // alternate the x-march direction step by step (STP_FD_SWEEP=up: always up)
     int nxh = 0, nyh = 0, nzh = 0;
     int64_t ncell = 0;
     float cx[5] = {0}, cy[5] = {0}, cz[5] = {0};

@@ -811,7 +811,7 @@ def main() -> int:
                       "amplitude_limit_reached": bool(field_max >= 0.99e10)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "fd_step_kernel<4,4,128,16,1,256,4,2,true>", "bytes_per_cell_update": BYTES_PER_CELL,
+                     "kernel": "fd_step_kernel<4,4,128,16,1,256,4,2,6>", "bytes_per_cell_update": BYTES_PER_CELL,
                      "avg_launch_us": round(launch_s * 1e6, 2), "traffic_source": traffic_src},
     }
     copy_gbs = stream_copy_rate(dev)

@@ -277,3 +277,63 @@ def test_tile_variants_guards_and_status(variant, cuda):
     got = fd.cur[0].cpu().numpy()
     assert np.isfinite(got).all()
     assert rel_l2(got, cur_o) < TOL
+
+
+def _with_sweep(sweep, fn):
+    import os
+    old = os.environ.pop("STP_FD_SWEEP", None)
+    if sweep is not None:
+        os.environ["STP_FD_SWEEP"] = sweep
+    try:
+        return fn()
+    finally:
+        os.environ.pop("STP_FD_SWEEP", None)
+        if old is not None:
+            os.environ["STP_FD_SWEEP"] = old
+
+
+@pytest.mark.parametrize("n,chunks,nshots", [
+    ((300, 36, 136), (7,), 1),         # two x chunks with tails, an odd step count
+    ((70, 45, 68), (6, 11, 23), 1),    # calls that start on odd and even steps
+    ((36, 30, 28), (9,), 2),           # batched shots
+    ((3, 30, 28), (5,), 1),            # a chunk shorter than the halo width
+])
+def test_alternating_sweeps_equal_one_way_sweeps(n, chunks, nshots, cuda):
+    """Steps march x upward and downward by turns (so each starts on the
+    planes the previous one left in the caches); the x taps commute, so the
+    fields and traces equal those of steps that always march upward, bit for
+    bit."""
+    import torch
+    nt = sum(chunks)
+    alt = _with_sweep(None, lambda: _run_fd(cuda, n, nt, chunks, None, nshots=nshots))
+    up = _with_sweep("up", lambda: _run_fd(cuda, n, nt, chunks, None, nshots=nshots))
+    for a, b in zip(alt, up):
+        assert torch.equal(a, b)
+
+
+def test_alternating_sweeps_in_the_adjoint_passes(cuda):
+    """The checkpointed forward pass, the snapshot recompute and the reverse
+    step alternate their sweeps as well: image, illumination and traces
+    equal those of one-way sweeps, bit for bit."""
+    import torch
+    n, nt = (40, 28, 32), 37
+    vel = synthetic.smooth_random_velocity(*n, seed=3, sigma=3.0, vmin=1800.0, vmax=2600.0)
+    w = synthetic.ricker(25.0, 1e-3, nt)
+
+    def run():
+        fd = FDPropagator(*n, order=8, sponge_width=6, device=cuda, sponge="cerjan")
+        fd.set_velocity(vel)
+        fd.set_sources([fd.index(n[0] // 2, n[1] // 2, 3)])
+        fd.set_receivers(synthetic.surface_receivers(*n, 4, iz=1))
+        obs = fd.propagate(nt, w).clone()
+        fd.reset()
+        rtm = fd.reverse_time(nt, w, inject=obs, snapshot="field", illumination=True)   # image + illumination
+        grad = fd.reverse_time(nt, w, obs=obs * 0.5)                                       # residual injection
+        tensors = [o[k].clone() for o in (rtm, grad) for k in ("acc", "ill", "traces") if o[k] is not None]
+        return tensors, grad["misfit"]
+
+    (alt, m_alt), (up, m_up) = _with_sweep(None, run), _with_sweep("up", run)
+    assert len(alt) == len(up) == 5
+    for a, b in zip(alt, up):
+        assert torch.equal(a, b)
+    assert m_alt == m_up
