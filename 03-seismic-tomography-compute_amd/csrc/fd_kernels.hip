@@ -686,6 +686,8 @@ static hipError_t launch_cfg(const FdStepArgs& a0, int nblocks, int mode, hipStr
 // interleaved on one box: entry 0 381.3 us/step, 1 389.0, 3 ~411; on others
 // 384.1-385.5 (entry 0) against 382.3-383.0 (entry 2), and 378.5-381.9
 // against 381.4-386.2 in full propagations: within the spread of the boxes.
+// These are upward-only sweeps; with the alternating sweeps the host uses,
+// on one box: entries 0-3 378.2, 385.6, 377.8 and 404.4 us.
 // Measured and not kept: three planes ahead with a guarded tail (390.3-390.8),
 // P, C and N all non-temporal (388.5, against 389.0 for entry 1), late halos
 // two planes ahead (398), 128 x 8 cells (422), 256 x 8 cells (397), shorter
