@@ -202,7 +202,7 @@ __global__ void finalize_coef_kernel(int n0, int n1, int n2, const float* __rest
 struct Cell {
     int64_t i, V;
     int64_t st[3];
-    bool lo[3];
+    bool lo[3], hi[3];
 };
 __device__ __forceinline__ Cell cell_at(int64_t i, int n0, int n1, int n2) {
     Cell c;
@@ -215,36 +215,72 @@ __device__ __forceinline__ Cell cell_at(int64_t i, int n0, int n1, int n2) {
     c.lo[0] = x > 0;
     c.lo[1] = y > 0;
     c.lo[2] = z > 0;
+    c.hi[0] = x + 1 < n0;
+    c.hi[1] = y + 1 < n1;
+    c.hi[2] = z + 1 < n2;
     return c;
 }
 
 // ((I + L_w) (s .* v))_i for the cell c; with PERM the value of cell j is
-// v[perm[j]]
+// v[perm[j]].  Every operand is loaded before the first term is formed (a
+// missing neighbour reads cell i and its term is skipped), so a thread waits
+// for its loads once instead of once per edge; the terms are added in the
+// order +x, -x, +y, -y, +z, -z.
 template <bool PERM = false>
 __device__ __forceinline__ float graph_apply(const Cell& c, const float* __restrict__ coef,
-                                             const float* __restrict__ v, const int* __restrict__ perm = nullptr) {
-    auto at = [&](int64_t j) { return PERM ? v[perm[j]] : v[j]; };
-    const float yi = coef[c.i] * at(c.i);
+                                             const float* __restrict__ v, const int* __restrict__ perm = nullptr,
+                                             float* vi_out = nullptr) {
+    int64_t jh[3], jl[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        jh[d] = c.hi[d] ? c.i + c.st[d] : c.i;
+        jl[d] = c.lo[d] ? c.i - c.st[d] : c.i;
+    }
+    int64_t ki = c.i, kh[3], kl[3];   // positions of the values in v
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        kh[d] = jh[d];
+        kl[d] = jl[d];
+    }
+    if constexpr (PERM) {
+        ki = perm[c.i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            kh[d] = perm[jh[d]];
+            kl[d] = perm[jl[d]];
+        }
+    }
+    float wh[3], wl[3], sh[3], sl[3], vh[3], vl[3];
+    const float si = coef[c.i], vi = v[ki];
+    if (vi_out) *vi_out = vi;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wh[d] = coef[(1 + d) * c.V + c.i];
+        wl[d] = coef[(1 + d) * c.V + jl[d]];
+        sh[d] = coef[jh[d]];
+        sl[d] = coef[jl[d]];
+        vh[d] = v[kh[d]];
+        vl[d] = v[kl[d]];
+    }
+    const float yi = si * vi;
     float u = yi;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const float wf = coef[(1 + d) * c.V + c.i];
-        if (wf != 0.f) {
-            const int64_t j = c.i + c.st[d];
-            u += wf * (yi - coef[j] * at(j));
-        }
-        if (c.lo[d]) {
-            const int64_t j = c.i - c.st[d];
-            u += coef[(1 + d) * c.V + j] * (yi - coef[j] * at(j));
-        }
+        // The differences are formed for both edges whether or not they
+        // exist (the empty asm keeps the compiler from moving their loads
+        // into a branch), then added where the edge exists.
+        float th = yi - sh[d] * vh[d], tl = yi - sl[d] * vl[d];
+        asm volatile("" : "+v"(th), "+v"(tl));
+        if (wh[d] != 0.f) u += wh[d] * th;   // w is zero at the far face
+        if (c.lo[d]) u += wl[d] * tl;
     }
     return u;
 }
 
-// z = M r, optionally with the two FP64 dot partials of flexible CGLS
-// (z.r and z.(r - r_prev), one per block of 256 cells, like flex_dots_kernel).
-// With PERM, r, zo and r_prev hold cell i's value at perm[i].
-template <bool PERM>
+// z = M r, with DOTS also the two FP64 dot partials of flexible CGLS (z.r and
+// z.(r - r_prev), one per block of 256 cells, like flex_dots_kernel).  With
+// PERM, r, zo and r_prev hold cell i's value at perm[i].
+template <bool PERM, bool DOTS>
 __global__ void __launch_bounds__(256) apply_m_kernel(int n0, int n1, int n2, const float* __restrict__ coef,
                                                       const float* __restrict__ r, float* __restrict__ zo,
                                                       const int* done, const float* __restrict__ r_prev,
@@ -256,14 +292,16 @@ __global__ void __launch_bounds__(256) apply_m_kernel(int n0, int n1, int n2, co
     if (i < V) {
         const Cell c = cell_at(i, n0, n1, n2);
         const int64_t pi = PERM ? (int64_t)perm[i] : i;
-        const float z = coef[i] * graph_apply<PERM>(c, coef, r, perm);
+        float ri;   // r at cell i, from the stencil's own load
+        const float z = coef[i] * graph_apply<PERM>(c, coef, r, perm, &ri);
         zo[pi] = z;
-        if (part_zr) {
-            a = (double)z * (double)r[pi];
-            b = (double)z * ((double)r[pi] - (double)r_prev[pi]);
+        if constexpr (DOTS) {
+            const float rq = r_prev[pi];
+            a = (double)z * (double)ri;
+            b = (double)z * ((double)ri - (double)rq);
         }
     }
-    if (!part_zr) return;
+    if constexpr (!DOTS) return;
     __shared__ double red[2][256];
     red[0][threadIdx.x] = a;
     red[1][threadIdx.x] = b;
@@ -531,12 +569,17 @@ int lpc_apply(void* handle, const float* s, float* z, const int* done, const flo
     if ((part_zs || part_fl) && !(part_zs && part_fl && s_prev))
         return fail(STP_ERR_INVALID, "learned preconditioner: incomplete dot-product outputs");
     const int64_t V = h->V;
-    if (perm)
-        hipLaunchKernelGGL(apply_m_kernel<true>, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, h->n0, h->n1,
-                           h->n2, h->coef, s, z, done, s_prev, part_zs, part_fl, perm);
-    else
-        hipLaunchKernelGGL(apply_m_kernel<false>, dim3((unsigned)((V + 255) / 256)), dim3(256), 0, st, h->n0, h->n1,
-                           h->n2, h->coef, s, z, done, s_prev, part_zs, part_fl, nullptr);
+    const dim3 grid((unsigned)((V + 255) / 256));
+#define STP_APPLY_M(PERM, DOTS) hipLaunchKernelGGL((apply_m_kernel<PERM, DOTS>), grid, dim3(256), 0, st, h->n0, \
+        h->n1, h->n2, h->coef, s, z, done, s_prev, part_zs, part_fl, perm)
+    if (perm) {
+        if (part_zs) STP_APPLY_M(true, true);
+        else STP_APPLY_M(true, false);
+    } else {
+        if (part_zs) STP_APPLY_M(false, true);
+        else STP_APPLY_M(false, false);
+    }
+#undef STP_APPLY_M
     STP_HIP(hipGetLastError());
     return STP_OK;
 }

@@ -328,7 +328,22 @@ __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
         const int t = rev ? ntiles - 1 - t0 : t0;
         const int ub = tu[t], nu = tu[t + 1] - ub;
         __syncthreads();   // the previous tile's reads of xs are done
-        for (int k = threadIdx.x; k < nu; k += 256) xs[k] = r[uniq[ub + k]];
+        // Stage the tile's rays: SU list entries per thread are loaded
+        // together, then their gathers, so a thread waits two memory latencies
+        // per 256 SU rays instead of two per ray (slots past the list's end
+        // repeat its last entry and are not stored).
+        constexpr int SU = 8;
+        for (int k0 = threadIdx.x; k0 < nu; k0 += SU * 256) {
+            int id[SU];
+            float rv[SU];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) id[u] = uniq[ub + min(k0 + u * 256, nu - 1)];
+#pragma unroll
+            for (int u = 0; u < SU; ++u) rv[u] = r[id[u]];
+#pragma unroll
+            for (int u = 0; u < SU; ++u)
+                if (k0 + u * 256 < nu) xs[k0 + u * 256] = rv[u];
+        }
         __syncthreads();
         const int r1 = min(t * rt + rt, rows);
         for (int row = t * rt + sub; row < r1; row += RPB) {
@@ -378,11 +393,29 @@ __device__ void block_sums(const double* a, int na, const double* b, int nb, con
                            double* rb, double* rc) {
     __shared__ double red[3][4];
     double sa = 0.0, sb = 0.0, sc = 0.0;
-    for (int i = threadIdx.x; i < na; i += 256) sa += a[i];
-    if (b)
-        for (int i = threadIdx.x; i < nb; i += 256) sb += b[i];
-    if (c)
-        for (int i = threadIdx.x; i < nc; i += 256) sc += c[i];
+    // Thread t adds a[t], a[t + 256], ... in that order (likewise b and c).
+    // The loads of U strided entries of all three arrays go out before the
+    // adds, so a thread waits for one memory latency per 256 U partials
+    // instead of one per entry; the order of the adds is unchanged.
+    constexpr int U = 8;
+    const int nmax = na > nb ? (na > nc ? na : nc) : (nb > nc ? nb : nc);
+    for (int i0 = threadIdx.x; i0 < nmax; i0 += U * 256) {
+        double va[U], vb[U], vc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * 256;
+            va[u] = i < na ? a[i] : 0.0;
+            vb[u] = b && i < nb ? b[i] : 0.0;
+            vc[u] = c && i < nc ? c[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * 256;
+            if (i < na) sa += va[u];
+            if (b && i < nb) sb += vb[u];
+            if (c && i < nc) sc += vc[u];
+        }
+    }
     sa = wave_sum(sa);
     sb = wave_sum(sb);
     sc = wave_sum(sc);
@@ -403,6 +436,61 @@ __device__ double block_sum(const double* part, int n) {
     double r;
     block_sums(part, n, nullptr, 0, nullptr, 0, &r, nullptr, nullptr);
     return r;
+}
+
+// x += a p (cols), r -= a q and, with HY, y += a q (rows); v32 receives the
+// FP32 copy of y (HY) or r; |r|^2 and |y|^2 accumulate in rr and yy.  HC / HR:
+// x and p / r and q have at least one element.  A thread's U grid-strided
+// elements are loaded together (slots past an array's end load its last
+// element and leave it unused), then updated in ascending order, so the sums
+// keep the order of a plain strided loop.
+template <bool HC, bool HR, bool HY>
+__device__ __forceinline__ void cgls_update_loop(double a, int cols, int rows, double* __restrict__ x,
+                                                 const float* __restrict__ p, double* __restrict__ r,
+                                                 const float* __restrict__ q, double* __restrict__ y,
+                                                 float* __restrict__ v32, double& rr, double& yy) {
+    constexpr int U = 4;
+    const int n = cols > rows ? cols : rows;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += U * stride) {
+        double xv[U], rv[U], yv[U];
+        float pv[U], qv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * stride;
+            if constexpr (HC) {
+                const int ic = i < cols ? i : cols - 1;
+                xv[u] = x[ic];
+                pv[u] = p[ic];
+            }
+            if constexpr (HR) {
+                const int ir = i < rows ? i : rows - 1;
+                rv[u] = r[ir];
+                qv[u] = q[ir];
+                if constexpr (HY) yv[u] = y[ir];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * stride;
+            if constexpr (HC)
+                if (i < cols) x[i] = xv[u] + a * (double)pv[u];
+            if constexpr (HR)
+                if (i < rows) {
+                    const double rn = rv[u] - a * (double)qv[u];
+                    r[i] = rn;
+                    if constexpr (HY) {
+                        const double yn = yv[u] + a * (double)qv[u];
+                        y[i] = yn;
+                        if (v32) v32[i] = (float)yn;
+                        yy += yn * yn;
+                    } else if (v32) {
+                        v32[i] = (float)rn;
+                    }
+                    rr += rn * rn;
+                }
+        }
+    }
 }
 
 // CGLS, after q = A p (with its |q|^2 partials): alpha = gamma / (|q|^2 +
@@ -436,23 +524,15 @@ __global__ void __launch_bounds__(256) cgls_alpha_update_kernel(SolverScalars* s
     }
     const double a = sc->gam[it & 1] / del;
     if (first) sc->alpha = a;
-    const int n = cols > rows ? cols : rows;
     double rr = 0.0, yy = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (i < cols) x[i] += a * (double)p[i];
-        if (i < rows) {
-            const double rv = r[i] - a * (double)q[i];
-            r[i] = rv;
-            if (y) {
-                const double yv = y[i] + a * (double)q[i];
-                y[i] = yv;
-                if (v32) v32[i] = (float)yv;
-                yy += yv * yv;
-            } else if (v32) {
-                v32[i] = (float)rv;
-            }
-            rr += rv * rv;
-        }
+    if (cols > 0 && rows > 0) {
+        if (y) cgls_update_loop<true, true, true>(a, cols, rows, x, p, r, q, y, v32, rr, yy);
+        else cgls_update_loop<true, true, false>(a, cols, rows, x, p, r, q, y, v32, rr, yy);
+    } else if (cols > 0) {
+        cgls_update_loop<true, false, false>(a, cols, rows, x, p, r, q, y, v32, rr, yy);
+    } else if (rows > 0) {
+        if (y) cgls_update_loop<false, true, true>(a, cols, rows, x, p, r, q, y, v32, rr, yy);
+        else cgls_update_loop<false, true, false>(a, cols, rows, x, p, r, q, y, v32, rr, yy);
     }
     if (part_rr) {
         block_partial(rr, part_rr);
@@ -501,11 +581,27 @@ __global__ void __launch_bounds__(256) cgls_beta_direction_kernel(SolverScalars*
         }
     }
     double v = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cols; i += gridDim.x * blockDim.x) {
-        const float pv = (float)((double)z[i] + b * (double)p[i]);
-        p[i] = pv;
-        v += (double)pv * (double)pv;
-        if (s_prev) s_prev[i] = s[i];
+    constexpr int U = 4;   // loads of a thread's U strided elements together, as in the update kernel
+    const int stride = gridDim.x * blockDim.x;
+    for (int i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < cols; i0 += U * stride) {
+        float zv[U], pv[U], sv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {   // past the end: the last element, not used
+            const int ic = min(i0 + u * stride, cols - 1);
+            zv[u] = z[ic];
+            pv[u] = p[ic];
+            if (s_prev) sv[u] = s[ic];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * stride;
+            if (i < cols) {
+                const float pn = (float)((double)zv[u] + b * (double)pv[u]);
+                p[i] = pn;
+                v += (double)pn * (double)pn;
+                if (s_prev) s_prev[i] = sv[u];
+            }
+        }
     }
     block_partial(v, part_pp);
 }
