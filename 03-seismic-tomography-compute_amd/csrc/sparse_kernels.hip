@@ -35,11 +35,10 @@ constexpr int MAX_BLOCKS = SPMV_MAX_BLOCKS;
 // the row's first entry and contribute an exact zero, which leaves the sum
 // (and its order) unchanged.
 template <int W, typename AT, typename XT, typename VT = float, bool NOVAL = false, typename IT = int>
-__device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const IT* __restrict__ ci,
-                                      const VT* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
+__device__ __forceinline__ AT row_dot_range(int b, int e, const IT* __restrict__ ci, const VT* __restrict__ val,
+                                            const XT* __restrict__ x, int lane) {
     constexpr int U = (128 / W) < 1 ? 1 : (128 / W);
     AT acc = 0;
-    const int b = rp[row], e = rp[row + 1];
     for (int k0 = b + lane; k0 < e; k0 += U * W) {
         VT v[U];
         int c[U];
@@ -63,6 +62,13 @@ __device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const IT* __re
 #pragma unroll
     for (int m = W / 2; m > 0; m >>= 1) acc += __shfl_xor(acc, m);
     return acc;
+}
+
+// The same for row `row` of the CSR arrays (rp, ci, val).
+template <int W, typename AT, typename XT, typename VT = float, bool NOVAL = false, typename IT = int>
+__device__ __forceinline__ AT row_dot(const int* __restrict__ rp, const IT* __restrict__ ci,
+                                      const VT* __restrict__ val, const XT* __restrict__ x, int row, int lane) {
+    return row_dot_range<W, AT, XT, VT, NOVAL, IT>(rp[row], rp[row + 1], ci, val, x, lane);
 }
 
 // Block reductions for 256-thread blocks (4 wavefronts): a butterfly of
@@ -94,10 +100,22 @@ __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restri
     constexpr int RPB = 256 / W;
     const int lane = threadIdx.x % W, sub = threadIdx.x / W;
     double ps = 0.0;
+    // Offsets of the group's rows are loaded one row ahead (a group past the
+    // last row loads the last row's and does not use them).
+    auto offsets = [&](int row0, int& b, int& e) {
+        const int k = row0 < rows - sub ? row0 + sub : rows - 1;
+        const int row = rev ? rows - 1 - k : k;
+        b = rp[row];
+        e = rp[row + 1];
+    };
+    int b = 0, e = 0;
+    if (blockIdx.x * RPB < rows) offsets(blockIdx.x * RPB, b, e);
     for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
+        int nb, ne;
+        offsets(row0 < rows - (int)gridDim.x * RPB ? row0 + (int)gridDim.x * RPB : rows - 1, nb, ne);
         if (row0 + sub < rows) {
             const int row = rev ? rows - 1 - (row0 + sub) : row0 + sub;
-            const AT acc = row_dot<W, AT, XT, VT>(rp, ci, val, x, row, lane);
+            const AT acc = row_dot_range<W, AT, XT, VT>(b, e, ci, val, x, lane);
             if (lane == 0) {
                 double v = alpha * (double)acc;
                 if (FUSE_W) v -= lam * wv[row];
@@ -106,6 +124,8 @@ __global__ void __launch_bounds__(256) spmv_kernel(int rows, const int* __restri
                 if (PART) ps += (double)yv * (double)yv;
             }
         }
+        b = nb;
+        e = ne;
     }
     if (PART) block_partial(ps, part);
 }
@@ -312,7 +332,7 @@ hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, co
 // gather from LDS through 16-bit positions; the row order, lanes and chunking
 // are those of normal_residual_kernel, so each row's sum is the same.  As
 // there, with g the gathered vector is y and s = g - A^T y - lam x.
-template <int W>
+template <int W, bool HG, bool HD>
 __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
     int rows, int rt, int ntiles, const int* __restrict__ rp, const uint16_t* __restrict__ l16,
     const float* __restrict__ val, const int* __restrict__ tu, const int* __restrict__ uniq,
@@ -327,11 +347,21 @@ __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
     for (int t0 = blockIdx.x; t0 < ntiles; t0 += gridDim.x) {
         const int t = rev ? ntiles - 1 - t0 : t0;
         const int ub = tu[t], nu = tu[t + 1] - ub;
+        const int r1 = min(t * rt + rt, rows);
+        // Row offsets of the group's first row (rows t rt .. r1 - 1 are not
+        // empty), loaded with the staging loads.
+        int b, e;
+        {
+            const int rf = min(t * rt + sub, r1 - 1);
+            b = rp[rf];
+            e = rp[rf + 1];
+        }
         __syncthreads();   // the previous tile's reads of xs are done
         // Stage the tile's rays: SU list entries per thread are loaded
         // together, then their gathers, so a thread waits two memory latencies
-        // per 256 SU rays instead of two per ray (slots past the list's end
-        // repeat its last entry and are not stored).
+        // per 256 SU rays instead of two per ray.  Slots past the list's end
+        // take its last entry and store the same value to its slot (no
+        // branch, so the compiler keeps every load in the batch).
         constexpr int SU = 8;
         for (int k0 = threadIdx.x; k0 < nu; k0 += SU * 256) {
             int id[SU];
@@ -341,16 +371,22 @@ __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
 #pragma unroll
             for (int u = 0; u < SU; ++u) rv[u] = r[id[u]];
 #pragma unroll
-            for (int u = 0; u < SU; ++u)
-                if (k0 + u * 256 < nu) xs[k0 + u * 256] = rv[u];
+            for (int u = 0; u < SU; ++u) xs[min(k0 + u * 256, nu - 1)] = rv[u];
         }
         __syncthreads();
-        const int r1 = min(t * rt + rt, rows);
         for (int row = t * rt + sub; row < r1; row += RPB) {
-            const double acc = row_dot<W, double, float, float, false, uint16_t>(rp, l16, val, xs, row, lane);
+            // the next row's offsets and this row's epilogue operands go out
+            // before the row's own loads
+            const int rn = min(row + RPB, r1 - 1);
+            const int nb = rp[rn], ne = rp[rn + 1];
+            const double gr = HG ? g[row] : 0.0, xr = x[row];
+            const float dr = HD ? dinv[row] : 1.0f;
+            const double acc = row_dot_range<W, double, float, float, false, uint16_t>(b, e, l16, val, xs, lane);
+            b = nb;
+            e = ne;
             if (lane == 0) {
-                const float sv = (float)((g ? g[row] - acc : acc) - lam * x[row]);
-                const float zv = dinv ? dinv[row] * sv : sv;
+                const float sv = (float)((HG ? gr - acc : acc) - lam * xr);
+                const float zv = HD ? dr * sv : sv;
                 s[row] = sv;
                 z[row] = zv;
                 psz += (double)sv * (double)zv;
@@ -372,16 +408,23 @@ hipError_t normal_residual_staged(int W, int rows, int rt, int ntiles, int maxu,
     if (maxu < 0 || maxu > STAGE_CAP || rt < 1) return hipErrorInvalidValue;
     const int nb = staged_blocks(ntiles);
     const size_t lds = sizeof(float) * (size_t)(maxu > 0 ? maxu : 1);
+#define STP_STAGED(WW)                                                                                          \
+    do {                                                                                                        \
+        if (g && dinv) STP_STAGED_LAUNCH(WW, true, true);                                                       \
+        else if (g) STP_STAGED_LAUNCH(WW, true, false);                                                         \
+        else if (dinv) STP_STAGED_LAUNCH(WW, false, true);                                                      \
+        else STP_STAGED_LAUNCH(WW, false, false);                                                               \
+    } while (0)
+#define STP_STAGED_LAUNCH(WW, G, D) hipLaunchKernelGGL((normal_residual_staged_kernel<WW, G, D>), dim3(nb), dim3(256), \
+        lds, st, rows, rt, ntiles, rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev)
     switch (W) {
-        case 64: hipLaunchKernelGGL(normal_residual_staged_kernel<64>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
-        case 32: hipLaunchKernelGGL(normal_residual_staged_kernel<32>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
-        case 16: hipLaunchKernelGGL(normal_residual_staged_kernel<16>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
-        default: hipLaunchKernelGGL(normal_residual_staged_kernel<8>, dim3(nb), dim3(256), lds, st, rows, rt, ntiles,
-                                    rp, l16, val, tu, uniq, y, g, x, lam, dinv, s, z, part_sz, part_ss, done, rev); break;
+        case 64: STP_STAGED(64); break;
+        case 32: STP_STAGED(32); break;
+        case 16: STP_STAGED(16); break;
+        default: STP_STAGED(8); break;
     }
+#undef STP_STAGED_LAUNCH
+#undef STP_STAGED
     return hipGetLastError();
 }
 
