@@ -67,9 +67,16 @@ def test_conv3d_backward_matches_oracle(cin, cout, cuda):
     assert rel(dw.cpu(), wd.grad) < 1e-5
 
 
-def test_features_match_oracle(cuda):
-    grid = (8, 8, 8)
-    A, N = device_system(grid, 700, seed=11)
+@pytest.mark.parametrize("grid,nrays,rowlen", [
+    ((8, 8, 8), 700, (1, 256)),           # short cell rows, staged in LDS
+    ((4, 4, 4), 3000, (200, 1024)),       # rows of a few hundred rays, staged in LDS
+    ((4, 4, 4), 12000, (1025, 10 ** 6)),  # rows past 1024 rays: searched in global memory
+])
+def test_features_match_oracle(grid, nrays, rowlen, cuda):
+    A, N = device_system(grid, nrays, seed=11)
+    _, ci, _ = osp.ray_matrix(grid, 1.0, synthetic.ray_endpoints(nrays, grid, 1.0, seed=11))
+    per_cell = np.bincount(ci, minlength=int(np.prod(grid)))
+    assert rowlen[0] <= per_cell.max() <= rowlen[1], per_cell.max()
     net = LearnedPreconditioner(seed=0)
     f, d = net.features(A, grid, LAM)
     fo, do = ol.features(N, grid, LAM)
