@@ -361,6 +361,22 @@ struct CglsWork {
     double *part0 = nullptr, *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *part4 = nullptr,
            *part5 = nullptr, *pnorm = nullptr;   // pnorm: |r|^2 and |y|^2 partials, 2 x 256
     stp::SolverScalars* sc = nullptr;
+    // hipEvent_t for the host reads of the scalars (null: the reads synchronize
+    // the stream; the entry point that allocates nothing passes none)
+    hipEvent_t ev = nullptr;
+};
+
+// An event owned for the duration of one solve.
+struct OwnedEvent {
+    hipEvent_t ev = nullptr;
+    OwnedEvent() {
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+    }
+    ~OwnedEvent() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    OwnedEvent(const OwnedEvent&) = delete;
+    OwnedEvent& operator=(const OwnedEvent&) = delete;
 };
 
 int cgls_check(const stp_csr* h, const double* b, double lam, int32_t max_iter, int32_t precond, void* ph,
@@ -459,11 +475,13 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
         int improved_at = it;
         bool stalled = false;
         bool ymode = use32;
+        bool ap_queued = false;   // this iteration's A p was queued before the last host read
         for (; it < max_iter; ++it) {
             const int rev = it & 1;
             const bool check = (it + 1) % CHECK == 0 || it + 1 == max_iter;
             const bool norms = ymode && check;
-            STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st, rev));
+            if (!ap_queued) STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st, rev));
+            ap_queued = false;
             STP_HIP(stp::launch_cgls_alpha_update(w.sc, w.part0, nbq, w.part1, npp, lam, it, cols, x, w.p, rows, w.r,
                                                   w.q, ymode ? w.y : nullptr, use32 ? w.v32 : nullptr,
                                                   norms ? w.pnorm : nullptr, norms ? w.pnorm + npa : nullptr, st));
@@ -491,7 +509,18 @@ int cgls_solve(stp_csr* h, const double* b, double lam, double tol, int32_t max_
                 if (norms)
                     STP_HIP(hipMemcpyAsync(hnorm, w.pnorm, sizeof(double) * 2 * npa, hipMemcpyDeviceToHost,
                                            st));
-                STP_HIP(hipStreamSynchronize(st));
+                if (w.ev && it + 1 < max_iter) {
+                    // The next iteration's A p depends on nothing decided below
+                    // (after convergence it returns at once, after a stall its
+                    // q is not used), so it is queued before the host waits for
+                    // the copies and the device is not idle meanwhile.
+                    STP_HIP(hipEventRecord(w.ev, st));
+                    STP_HIP(stp::spmv_f32(W, rows, RP, CI, VAL, w.p, w.q, w.part0, done, st, (it + 1) & 1));
+                    ap_queued = true;
+                    STP_HIP(hipEventSynchronize(w.ev));
+                } else {
+                    STP_HIP(hipStreamSynchronize(st));
+                }
                 if (hs.done) break;
                 if (norms) {
                     double rr = 0.0, yy = 0.0;
@@ -632,6 +661,8 @@ int stp_cgls(const stp_csr* hc, const double* b, double lam, double tol, int32_t
     w.r = h->r; w.p = h->p; w.q = h->q; w.s = h->s; w.z = h->z; w.sp = h->sp; w.dinv = h->dinv; w.v32 = h->r32; w.y = h->ry; w.g = h->rg; w.pnorm = h->pnorm;
     w.part0 = h->part0; w.part1 = h->part1; w.part2 = h->part2; w.part3 = h->part3; w.part4 = h->part4;
     w.part5 = h->part5; w.sc = h->sc;
+    OwnedEvent ev;
+    w.ev = ev.ev;
     return cgls_solve(h, b, lam, tol, max_iter, precond, precond_handle, x, history, info, st, w);
 }
 
