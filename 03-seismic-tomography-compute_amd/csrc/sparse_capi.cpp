@@ -254,6 +254,19 @@ int stp_csr_set_values_f64(stp_csr* h, const double* val64, void* stream) {
     return STP_OK;
 }
 
+// An event owned for the duration of one solve.
+struct OwnedEvent {
+    hipEvent_t ev = nullptr;
+    OwnedEvent() {
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+    }
+    ~OwnedEvent() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    OwnedEvent(const OwnedEvent&) = delete;
+    OwnedEvent& operator=(const OwnedEvent&) = delete;
+};
+
 int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int32_t jacobi, double* x,
            double* history, stp_solve_info* info, void* stream) {
     stp_csr* h = const_cast<stp_csr*>(hc);
@@ -300,8 +313,11 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
     int it = 0;
     stp::SolverScalars hs;
     std::memset(&hs, 0, sizeof hs);
+    OwnedEvent ev;
+    bool ap_queued = false;   // this iteration's A p was queued before the last host read
     for (; it < max_iter; ++it) {
-        STP_HIP(matvec(h->dp, h->dap, done));
+        if (!ap_queued) STP_HIP(matvec(h->dp, h->dap, done));
+        ap_queued = false;
         STP_HIP(stp::launch_dot_partial(done, n, h->dp, h->dap, h->part0, st));
         STP_HIP(stp::launch_cg_alpha(h->sc, h->part0, nbd, st));
         STP_HIP(stp::launch_cg_update(h->sc, n, x, h->dr, h->dp, h->dap, dinv, h->dz, h->part0, h->part1, st));
@@ -309,7 +325,16 @@ int stp_cg(const stp_csr* hc, const double* b, int32_t max_iter, double tol, int
         STP_HIP(stp::launch_cg_direction(h->sc, n, z, h->dp, st));
         if ((it + 1) % CHECK == 0 || it + 1 == max_iter) {
             STP_HIP(hipMemcpyAsync(&hs, h->sc, sizeof hs, hipMemcpyDeviceToHost, st));
-            STP_HIP(hipStreamSynchronize(st));
+            if (ev.ev && it + 1 < max_iter) {
+                // the next A p is queued before the host waits (after
+                // convergence it returns at once on the done flag)
+                STP_HIP(hipEventRecord(ev.ev, st));
+                STP_HIP(matvec(h->dp, h->dap, done));
+                ap_queued = true;
+                STP_HIP(hipEventSynchronize(ev.ev));
+            } else {
+                STP_HIP(hipStreamSynchronize(st));
+            }
             if (hs.done) break;
         }
     }
@@ -364,19 +389,6 @@ struct CglsWork {
     // hipEvent_t for the host reads of the scalars (null: the reads synchronize
     // the stream; the entry point that allocates nothing passes none)
     hipEvent_t ev = nullptr;
-};
-
-// An event owned for the duration of one solve.
-struct OwnedEvent {
-    hipEvent_t ev = nullptr;
-    OwnedEvent() {
-        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
-    }
-    ~OwnedEvent() {
-        if (ev) (void)hipEventDestroy(ev);
-    }
-    OwnedEvent(const OwnedEvent&) = delete;
-    OwnedEvent& operator=(const OwnedEvent&) = delete;
 };
 
 int cgls_check(const stp_csr* h, const double* b, double lam, int32_t max_iter, int32_t precond, void* ph,
