@@ -331,7 +331,9 @@ hipError_t normal_residual_r32(int W, int rows, const int* rp, const int* ci, co
 // neighbouring lanes read neighbouring rays), and the rows' dot products then
 // gather from LDS through 16-bit positions; the row order, lanes and chunking
 // are those of normal_residual_kernel, so each row's sum is the same.  As
-// there, with g the gathered vector is y and s = g - A^T y - lam x.
+// there, with g the gathered vector is y and s = g - A^T y - lam x.  HG / HD:
+// g / the Jacobi scaling dinv are given (compile-time, so that no load sits
+// under a run-time condition that would make the compiler wait for it early).
 template <int W, bool HG, bool HD>
 __global__ void __launch_bounds__(256) normal_residual_staged_kernel(
     int rows, int rt, int ntiles, const int* __restrict__ rp, const uint16_t* __restrict__ l16,
