@@ -273,10 +273,21 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
     constexpr int RPB = 256 / W;
     const int lane = threadIdx.x % W, sub = threadIdx.x / W;
     double psz = 0.0, pss = 0.0;
+    // row offsets one row ahead, as in spmv_kernel
+    auto offsets = [&](int row0, int& b, int& e) {
+        const int k = row0 < rows - sub ? row0 + sub : rows - 1;
+        const int row = rev ? rows - 1 - k : k;
+        b = rp[row];
+        e = rp[row + 1];
+    };
+    int b = 0, e = 0;
+    if (blockIdx.x * RPB < rows) offsets(blockIdx.x * RPB, b, e);
     for (int row0 = blockIdx.x * RPB; row0 < rows; row0 += gridDim.x * RPB) {
+        int nb, ne;
+        offsets(row0 < rows - (int)gridDim.x * RPB ? row0 + (int)gridDim.x * RPB : rows - 1, nb, ne);
         if (row0 + sub < rows) {
             const int row = rev ? rows - 1 - (row0 + sub) : row0 + sub;
-            const double acc = row_dot<W, double, RT>(rp, ci, val, r, row, lane);
+            const double acc = row_dot_range<W, double, RT>(b, e, ci, val, r, lane);
             if (lane == 0) {
                 if (gout) gout[row] = acc;
                 const float sv = (float)((gin ? gin[row] - acc : acc) - lam * x[row]);
@@ -287,6 +298,8 @@ __global__ void __launch_bounds__(256) normal_residual_kernel(int rows, const in
                 pss += (double)sv * (double)sv;
             }
         }
+        b = nb;
+        e = ne;
     }
     block_partial(psz, part_sz);
     block_partial(pss, part_ss);
